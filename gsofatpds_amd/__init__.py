@@ -1,0 +1,368 @@
+"""MI355X symbolic-factorisation engine: Python view of the C ABI in ``include/gsofa.h``.
+
+The engine is ``libgsofa.so`` next to this file (HIP kernels for gfx950 plus host C++). This
+module only binds it with ctypes and converts arrays; every computation happens in the library.
+If the library is missing, every entry point raises ``RuntimeError``: there is no fallback.
+
+Interface mirror of the reference (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh):
+  * ``symbfact_min_id(argv, rank)``   <- ``symbfact_min_id(args, argv, myrank, fill, time, nsup, teps)`` (:2529)
+  * ``read_ascii(beg, end, csr)``      <- the reader inside ``symbfact_min_id`` (:2567-2616)
+  * ``Plan`` / ``Batch``               structured forms of the same computation
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsofa.so")
+CLI_PATH = os.path.join(_HERE, "gsofa_cli")
+
+# status codes (include/gsofa.h)
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_IO = 2
+ERR_BAD_CSR = 3
+ERR_HIP = 4
+ERR_NO_DEVICE = 5
+ERR_OUT_OF_MEMORY = 6
+ERR_CAPACITY = 7
+ERR_STATE = 8
+ERR_REJECTED = 9
+ERR_CANCELLED = 10
+ERR_NOT_READY = 11
+
+# workloads
+WL_GRID2D_4096 = 1
+WL_GRID3D_110592 = 2
+WL_GRID2D_659033 = 3
+WL_BATCH256 = 4
+WL_STREAM = 5
+
+GEN_STENCIL, GEN_ERDOS_RENYI = 0, 1
+ORDER_NATURAL, ORDER_ND, ORDER_RCM = 0, 1, 2
+DROP_ANY, DROP_LOWER, DROP_UPPER = 0, 1, 2
+
+
+class GsofaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gsofa error {code}: {msg}")
+        self.code = code
+
+
+class CSR(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz", C.c_int64), ("beg", C.POINTER(C.c_int32)),
+                ("end", C.POINTER(C.c_int32)), ("adj", C.POINTER(C.c_int32))]
+
+
+class Options(C.Structure):
+    _fields_ = [("chunk_size", C.c_int32), ("n_processes", C.c_int32), ("structure", C.c_int32),
+                ("device", C.c_int32), ("heavy_threads", C.c_int32), ("heavy_wgs_per_cu", C.c_int32),
+                ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("fill_count", C.c_uint64), ("nnz_F", C.c_uint64), ("nnz_L", C.c_uint64),
+                ("nnz_U", C.c_uint64), ("edge_checks", C.c_uint64), ("reach", C.c_uint64),
+                ("supernodes", C.c_int64), ("supernodes_counted", C.c_int64),
+                ("heavy_sources", C.c_uint64), ("time_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib_handle = None
+
+_SIGNATURES = {
+    "gsofa_last_error": (C.c_char_p, []),
+    "gsofa_generate_workload": (C.c_int, [C.c_int, C.c_int64, C.POINTER(C.c_void_p)]),
+    "gsofa_generate": (C.c_int, [C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gsofa_host_matrix_csr": (C.c_int, [C.c_void_p, C.POINTER(CSR)]),
+    "gsofa_host_matrix_free": (None, [C.c_void_p]),
+    "gsofa_read_ascii": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "gsofa_write_ascii": (C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p, C.c_char_p]),
+    "gsofa_symbfact_min_id": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+    "gsofa_device_query": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.c_char_p, C.c_int32]),
+    "gsofa_options_default": (None, [C.POINTER(Options)]),
+    "gsofa_plan_create": (C.c_int, [C.POINTER(CSR), C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "gsofa_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gsofa_sync": (C.c_int, [C.c_void_p]),
+    "gsofa_result_get": (C.c_int, [C.c_void_p, C.POINTER(Result)]),
+    "gsofa_nzu_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "gsofa_supernodes_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "gsofa_structure_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "gsofa_structure_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "gsofa_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "gsofa_work_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]),
+    "gsofa_plan_destroy": (None, [C.c_void_p]),
+    "gsofa_batch_create": (C.c_int, [C.POINTER(CSR), C.c_int32, C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "gsofa_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gsofa_batch_sync": (C.c_int, [C.c_void_p]),
+    "gsofa_batch_result_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Result)]),
+    "gsofa_batch_nzu_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "gsofa_batch_supernodes_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]),
+    "gsofa_batch_destroy": (None, [C.c_void_p]),
+}
+
+
+def lib() -> C.CDLL:
+    """The loaded engine library; raises if it was not built."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gsofatpds_amd.build` "
+                               "(the engine has no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise GsofaError(rc, lib().gsofa_last_error().decode(errors="replace"))
+
+
+def _i32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+@dataclass
+class Pattern:
+    """An int32 pattern in the reference's layout: row v is adj[beg[v]:end[v]]."""
+    n: int
+    beg: np.ndarray
+    end: np.ndarray
+    adj: np.ndarray
+
+    @classmethod
+    def from_rowptr(cls, rowptr: np.ndarray, adj: np.ndarray) -> "Pattern":
+        rowptr = np.asarray(rowptr, dtype=np.int64)
+        return cls(len(rowptr) - 1, rowptr[:-1].astype(np.int32), rowptr[1:].astype(np.int32),
+                   np.ascontiguousarray(adj, dtype=np.int32))
+
+    @classmethod
+    def from_dense(cls, dense: np.ndarray) -> "Pattern":
+        n = dense.shape[0]
+        rows = [np.nonzero(dense[i])[0].astype(np.int32) for i in range(n)]
+        ptr = np.zeros(n + 1, np.int64)
+        ptr[1:] = np.cumsum([len(r) for r in rows])
+        adj = np.concatenate(rows) if n else np.zeros(0, np.int32)
+        return cls.from_rowptr(ptr, adj)
+
+    def csr(self) -> CSR:
+        self.beg = np.ascontiguousarray(self.beg, dtype=np.int32)
+        self.end = np.ascontiguousarray(self.end, dtype=np.int32)
+        self.adj = np.ascontiguousarray(self.adj, dtype=np.int32)
+        return CSR(self.n, len(self.adj), _i32p(self.beg), _i32p(self.end), _i32p(self.adj))
+
+    @property
+    def nnz(self) -> int:
+        return int(len(self.adj))
+
+
+def _take_host_matrix(handle: C.c_void_p) -> Pattern:
+    L = lib()
+    view = CSR()
+    try:
+        _check(L.gsofa_host_matrix_csr(handle, C.byref(view)))
+        n, nnz = view.n, view.nnz
+        beg = np.ctypeslib.as_array(view.beg, (n,)).copy()
+        end = np.ctypeslib.as_array(view.end, (n,)).copy()
+        adj = np.ctypeslib.as_array(view.adj, (nnz,)).copy() if nnz else np.zeros(0, np.int32)
+    finally:
+        L.gsofa_host_matrix_free(handle)
+    return Pattern(n, beg, end, adj)
+
+
+def generate_workload(workload: int, member: int = 0) -> Pattern:
+    h = C.c_void_p()
+    _check(lib().gsofa_generate_workload(workload, member, C.byref(h)))
+    return _take_host_matrix(h)
+
+
+def generate(kind=GEN_STENCIL, dims=2, points=5, nx=0, ny=0, nz=1, n_keep=0, order=ORDER_NATURAL, nd_leaf=64,
+             drop_side=DROP_ANY, drop_ppm=0, extra_every=0, avg_degree_x100=600, seed=1) -> Pattern:
+    params = (C.c_int64 * 13)(kind, dims, points, nx, ny, nz, n_keep, order, nd_leaf, drop_side, drop_ppm,
+                              extra_every, avg_degree_x100)
+    h = C.c_void_p()
+    _check(lib().gsofa_generate(params, seed, C.byref(h)))
+    return _take_host_matrix(h)
+
+
+def read_ascii(beg_file: str, end_file: str, csr_file: str) -> Pattern:
+    h = C.c_void_p()
+    _check(lib().gsofa_read_ascii(beg_file.encode(), end_file.encode(), csr_file.encode(), C.byref(h)))
+    return _take_host_matrix(h)
+
+
+def write_ascii(p: Pattern, beg_file: str, end_file: str, csr_file: str) -> None:
+    c = p.csr()
+    _check(lib().gsofa_write_ascii(C.byref(c), beg_file.encode(), end_file.encode(), csr_file.encode()))
+
+
+def symbfact_min_id(argv: Sequence[str], rank: int = 0):
+    """Mirror of the reference entry: returns (fill_count, time_ms, n_supernodes, edge_checks).
+
+    ``argv`` is the full vector including argv[0] (11 strings), as in the reference.
+    """
+    L = lib()
+    enc = [a.encode() if isinstance(a, str) else a for a in argv]
+    arr = (C.c_char_p * len(enc))(*enc)
+    fill = C.c_uint64(0)
+    t = C.c_double(0.0)
+    ns = C.c_int64(0)
+    teps = C.c_uint64(0)
+    _check(L.gsofa_symbfact_min_id(len(enc), arr, rank, C.byref(fill), C.byref(t), C.byref(ns), C.byref(teps)))
+    return fill.value, t.value, ns.value, teps.value
+
+
+def device_query(device: int = 0) -> dict:
+    cus = C.c_int32()
+    lb = C.c_int64()
+    lc = C.c_int64()
+    arch = C.create_string_buffer(128)
+    _check(lib().gsofa_device_query(device, C.byref(cus), C.byref(lb), C.byref(lc), arch, 128))
+    return {"cus": cus.value, "lds_per_block": lb.value, "lds_per_cu": lc.value, "arch": arch.value.decode()}
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().gsofa_options_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, int(v))
+    return o
+
+
+def _stream_handle(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+class Plan:
+    """One matrix: ``gsofa_plan_*``. Keeps the pattern arrays alive while the plan exists."""
+
+    def __init__(self, pattern: Pattern, **options):
+        self.pattern = pattern
+        self.options = default_options(**options)
+        self._csr = pattern.csr()
+        h = C.c_void_p()
+        _check(lib().gsofa_plan_create(C.byref(self._csr), C.byref(self.options), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gsofa_plan_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, stream=None) -> "Plan":
+        _check(lib().gsofa_run(self._h, _stream_handle(stream)))
+        return self
+
+    def sync(self) -> "Plan":
+        _check(lib().gsofa_sync(self._h))
+        return self
+
+    def result(self) -> Result:
+        r = Result()
+        _check(lib().gsofa_result_get(self._h, C.byref(r)))
+        return r
+
+    def nzu(self) -> np.ndarray:
+        out = np.zeros(self.pattern.n, np.int32)
+        _check(lib().gsofa_nzu_get(self._h, _i32p(out)))
+        return out
+
+    def supernode_flags(self) -> np.ndarray:
+        out = np.zeros(self.pattern.n, np.uint8)
+        _check(lib().gsofa_supernodes_get(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * 4)()
+        _check(lib().gsofa_stage_times(self._h, ms))
+        return {"prep_ms": ms[0], "light_ms": ms[1], "heavy_ms": ms[2], "supernodes_ms": ms[3]}
+
+    def work_split(self) -> dict:
+        w, r, e = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().gsofa_work_split(self._h, C.byref(w), C.byref(r), C.byref(e)))
+        return {"heavy_edge_checks": w.value, "heavy_reach": r.value, "heavy_entries": e.value}
+
+    def structure(self):
+        """(u_rowptr, u_col, l_colptr, l_row): strict U by rows, strict L by columns, sorted."""
+        nu, nl = C.c_int64(), C.c_int64()
+        _check(lib().gsofa_structure_size(self._h, C.byref(nu), C.byref(nl)))
+        n = self.pattern.n
+        up = np.zeros(n + 1, np.int64)
+        lp = np.zeros(n + 1, np.int64)
+        ui = np.zeros(max(nu.value, 1), np.int32)
+        li = np.zeros(max(nl.value, 1), np.int32)
+        _check(lib().gsofa_structure_get(self._h, up.ctypes.data_as(C.POINTER(C.c_int64)), _i32p(ui),
+                                         lp.ctypes.data_as(C.POINTER(C.c_int64)), _i32p(li)))
+        return up, ui[:nu.value], lp, li[:nl.value]
+
+
+class Batch:
+    """Many matrices in one captured graph: ``gsofa_batch_*``."""
+
+    def __init__(self, patterns: Sequence[Pattern], **options):
+        self.patterns = list(patterns)
+        self.options = default_options(**options)
+        self._csrs = (CSR * len(self.patterns))(*[p.csr() for p in self.patterns])
+        h = C.c_void_p()
+        _check(lib().gsofa_batch_create(self._csrs, len(self.patterns), C.byref(self.options), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gsofa_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def run(self, stream=None) -> "Batch":
+        _check(lib().gsofa_batch_run(self._h, _stream_handle(stream)))
+        return self
+
+    def sync(self) -> "Batch":
+        _check(lib().gsofa_batch_sync(self._h))
+        return self
+
+    def result(self, i: int) -> Result:
+        r = Result()
+        _check(lib().gsofa_batch_result_get(self._h, i, C.byref(r)))
+        return r
+
+    def nzu(self, i: int) -> np.ndarray:
+        out = np.zeros(self.patterns[i].n, np.int32)
+        _check(lib().gsofa_batch_nzu_get(self._h, i, _i32p(out)))
+        return out
+
+    def supernode_flags(self, i: int) -> np.ndarray:
+        out = np.zeros(self.patterns[i].n, np.uint8)
+        _check(lib().gsofa_batch_supernodes_get(self._h, i, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out

@@ -1,0 +1,488 @@
+// C ABI of the engine: plans, batches, the reference's ASCII files and the drop-in entry.
+#include <hip/hip_runtime_api.h>
+
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "engine.h"
+
+using namespace gsofa;
+
+namespace {
+
+Options to_options(const gsofa_options* o) {
+  Options r;
+  if (!o) return r;
+  r.chunk = o->chunk_size;
+  r.n_processes = o->n_processes;
+  r.structure = o->structure;
+  r.device = o->device;
+  r.heavy_threads = o->heavy_threads;
+  r.heavy_wgs_per_cu = o->heavy_wgs_per_cu;
+  r.light_wgs_per_cu = o->light_wgs_per_cu;
+  r.qcap = o->queue_capacity;
+  return r;
+}
+
+void fill_result(const MatResult& r, const Options& opt, int32_t n, double ms, gsofa_result* out) {
+  out->fill_count = r.fill;
+  out->nnz_L = r.nnz_l;
+  out->nnz_U = r.nnz_u;
+  out->nnz_F = r.nnz_l + r.nnz_u + (uint64_t)n;
+  out->edge_checks = r.W;
+  out->reach = r.R;
+  out->supernodes_counted = r.nsup;
+  out->supernodes = r.nsup + opt.n_processes;
+  out->heavy_sources = r.heavy;
+  out->time_ms = ms;
+}
+
+// Events and stream bookkeeping shared by plans and batches.
+struct Timing {
+  hipEvent_t start = nullptr, stop = nullptr;
+  hipStream_t stream = nullptr;
+  bool pending = false;
+  double last_ms = 0.0;
+  int init() {
+    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess)
+      return fail(GSOFA_ERR_HIP, "hipEventCreate");
+    return GSOFA_OK;
+  }
+  ~Timing() {
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+  int finish() {
+    if (!pending) return GSOFA_OK;
+    hipError_t e = hipEventSynchronize(stop);
+    if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, start, stop) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventElapsedTime");
+    last_ms = ms;
+    pending = false;
+    return GSOFA_OK;
+  }
+};
+
+}  // namespace
+
+struct gsofa_plan {
+  Prepared prep;
+  DeviceSet set;
+  Timing timing;
+  hipEvent_t stage[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  ~gsofa_plan() {
+    for (hipEvent_t e : stage)
+      if (e) (void)hipEventDestroy(e);
+  }
+  std::vector<MatResult> results;
+  bool have_result = false;
+};
+
+struct gsofa_batch {
+  std::vector<Prepared> preps;
+  DeviceSet set;
+  Timing timing;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<MatResult> results;
+  bool have_result = false;
+  ~gsofa_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+};
+
+extern "C" {
+
+int gsofa_device_query(int device, int32_t* cus, int64_t* lds_per_block, int64_t* lds_per_cu, char* arch,
+                       int32_t arch_len) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(GSOFA_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= count) return fail(GSOFA_ERR_NO_DEVICE, "device ordinal out of range");
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipGetDeviceProperties");
+  if (cus) *cus = p.multiProcessorCount;
+  if (lds_per_block) *lds_per_block = (int64_t)p.sharedMemPerBlock;
+  if (lds_per_cu) *lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+  if (arch && arch_len > 0) {
+    std::strncpy(arch, p.gcnArchName, (size_t)arch_len - 1);
+    arch[arch_len - 1] = '\0';
+  }
+  return GSOFA_OK;
+}
+
+void gsofa_options_default(gsofa_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->chunk_size = 128;
+  o->n_processes = 1;
+}
+
+int gsofa_plan_create(const gsofa_csr* a, const gsofa_options* opt, gsofa_plan** out) {
+  if (!out) return fail(GSOFA_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  gsofa_options def;
+  gsofa_options_default(&def);
+  const Options o = to_options(opt ? opt : &def);
+  std::unique_ptr<gsofa_plan> p(new (std::nothrow) gsofa_plan);
+  if (!p) return fail(GSOFA_ERR_OUT_OF_MEMORY, "plan");
+  int rc = prepare(a, p->prep);
+  if (rc != GSOFA_OK) return rc;
+  rc = p->set.create({&p->prep}, o);
+  if (rc != GSOFA_OK) return rc;
+  rc = p->timing.init();
+  if (rc != GSOFA_OK) return rc;
+  for (hipEvent_t& e : p->stage)
+    if (hipEventCreate(&e) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventCreate");
+  *out = p.release();
+  return GSOFA_OK;
+}
+
+int gsofa_run(gsofa_plan* p, void* hip_stream) {
+  if (!p) return fail(GSOFA_ERR_INVALID_ARG, "plan is null");
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  if (p->set.options().structure && !p->set.structure_ready()) {
+    // sizing pass: counts give the exact row / column offsets of the structure
+    int rc = p->set.enqueue(s);
+    if (rc != GSOFA_OK) return rc;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("sizing pass: ") + hipGetErrorString(e));
+    rc = p->set.size_pools_from_counts();
+    if (rc != GSOFA_OK) return rc;
+  }
+  if (hipEventRecord(p->timing.start, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
+  int rc = p->set.enqueue(s, p->stage);
+  if (rc != GSOFA_OK) return rc;
+  if (hipEventRecord(p->timing.stop, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
+  p->timing.stream = s;
+  p->timing.pending = true;
+  p->have_result = false;
+  return GSOFA_OK;
+}
+
+int gsofa_sync(gsofa_plan* p) {
+  if (!p) return fail(GSOFA_ERR_INVALID_ARG, "plan is null");
+  int rc = p->timing.finish();
+  if (rc != GSOFA_OK) return rc;
+  rc = p->set.read_results(p->results);
+  if (rc != GSOFA_OK) return rc;
+  p->have_result = true;
+  return GSOFA_OK;
+}
+
+int gsofa_result_get(gsofa_plan* p, gsofa_result* out) {
+  if (!p || !out) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  fill_result(p->results[0], p->set.options(), p->prep.n, p->timing.last_ms, out);
+  return GSOFA_OK;
+}
+
+int gsofa_nzu_get(gsofa_plan* p, int32_t* nzu) {
+  if (!p || !nzu) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  return p->set.read_counts(0, nullptr, nullptr, nullptr, nzu);
+}
+
+int gsofa_supernodes_get(gsofa_plan* p, uint8_t* flags) {
+  if (!p || !flags) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  return p->set.read_counts(0, nullptr, nullptr, flags, nullptr);
+}
+
+int gsofa_structure_size(gsofa_plan* p, int64_t* nnz_u, int64_t* nnz_l) {
+  if (!p || !nnz_u || !nnz_l) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->set.structure_ready()) return fail(GSOFA_ERR_STATE, "structure not produced: enable it and run first");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  *nnz_u = (int64_t)p->results[0].nnz_u;
+  *nnz_l = (int64_t)p->results[0].nnz_l;
+  return GSOFA_OK;
+}
+
+int gsofa_structure_get(gsofa_plan* p, int64_t* u_rowptr, int32_t* u_col, int64_t* l_colptr, int32_t* l_row) {
+  if (!p || !u_rowptr || !l_colptr) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  std::vector<int64_t> up, lp;
+  std::vector<int32_t> ui, li;
+  int rc = p->set.read_structure(0, up, ui, lp, li);
+  if (rc != GSOFA_OK) return rc;
+  std::memcpy(u_rowptr, up.data(), up.size() * 8);
+  std::memcpy(l_colptr, lp.data(), lp.size() * 8);
+  if (!ui.empty()) {
+    if (!u_col) return fail(GSOFA_ERR_INVALID_ARG, "u_col is null");
+    std::memcpy(u_col, ui.data(), ui.size() * 4);
+  }
+  if (!li.empty()) {
+    if (!l_row) return fail(GSOFA_ERR_INVALID_ARG, "l_row is null");
+    std::memcpy(l_row, li.data(), li.size() * 4);
+  }
+  return GSOFA_OK;
+}
+
+int gsofa_stage_times(gsofa_plan* p, double* ms) {
+  if (!p || !ms) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  for (int k = 0; k < 4; ++k) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, p->stage[k], p->stage[k + 1]) != hipSuccess)
+      return fail(GSOFA_ERR_HIP, "hipEventElapsedTime");
+    ms[k] = t;
+  }
+  return GSOFA_OK;
+}
+
+int gsofa_work_split(gsofa_plan* p, uint64_t* heavy_edge_checks, uint64_t* heavy_reach, uint64_t* heavy_entries) {
+  if (!p || !heavy_edge_checks || !heavy_reach || !heavy_entries) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  *heavy_edge_checks = p->results[0].heavy_w;
+  *heavy_reach = p->results[0].heavy_r;
+  *heavy_entries = p->results[0].heavy_nnz;
+  return GSOFA_OK;
+}
+
+void gsofa_plan_destroy(gsofa_plan* p) { delete p; }
+
+// ---- batches ---------------------------------------------------------------------------------
+
+int gsofa_batch_create(const gsofa_csr* mats, int32_t count, const gsofa_options* opt, gsofa_batch** out) {
+  if (!out || !mats || count <= 0) return fail(GSOFA_ERR_INVALID_ARG, "bad batch arguments");
+  *out = nullptr;
+  gsofa_options def;
+  gsofa_options_default(&def);
+  Options o = to_options(opt ? opt : &def);
+  if (o.structure) return fail(GSOFA_ERR_INVALID_ARG, "batches report counts and supernodes; structure is per plan");
+  std::unique_ptr<gsofa_batch> b(new (std::nothrow) gsofa_batch);
+  if (!b) return fail(GSOFA_ERR_OUT_OF_MEMORY, "batch");
+  b->preps.resize(count);
+  std::vector<const Prepared*> ptrs(count);
+  for (int32_t i = 0; i < count; ++i) {
+    int rc = prepare(&mats[i], b->preps[i]);
+    if (rc != GSOFA_OK) return fail(rc, "batch member " + std::to_string(i) + ": " + gsofa_last_error());
+    ptrs[i] = &b->preps[i];
+  }
+  int rc = b->set.create(ptrs, o);
+  if (rc != GSOFA_OK) return rc;
+  rc = b->timing.init();
+  if (rc != GSOFA_OK) return rc;
+  if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(GSOFA_ERR_HIP, "hipStreamCreate");
+  *out = b.release();
+  return GSOFA_OK;
+}
+
+int gsofa_batch_run(gsofa_batch* b, void* hip_stream) {
+  if (!b) return fail(GSOFA_ERR_INVALID_ARG, "batch is null");
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  if (!b->exec) {
+    if (hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(GSOFA_ERR_HIP, "hipStreamBeginCapture");
+    int rc = b->set.enqueue(b->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+    if (rc != GSOFA_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  if (hipEventRecord(b->timing.start, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
+  hipError_t e = hipGraphLaunch(b->exec, s);
+  if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+  if (hipEventRecord(b->timing.stop, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
+  b->timing.stream = s;
+  b->timing.pending = true;
+  b->have_result = false;
+  return GSOFA_OK;
+}
+
+int gsofa_batch_sync(gsofa_batch* b) {
+  if (!b) return fail(GSOFA_ERR_INVALID_ARG, "batch is null");
+  int rc = b->timing.finish();
+  if (rc != GSOFA_OK) return rc;
+  rc = b->set.read_results(b->results);
+  if (rc != GSOFA_OK) return rc;
+  b->have_result = true;
+  return GSOFA_OK;
+}
+
+int gsofa_batch_result_get(gsofa_batch* b, int32_t index, gsofa_result* out) {
+  if (!b || !out) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (index < 0 || index >= b->set.nmats()) return fail(GSOFA_ERR_INVALID_ARG, "batch index out of range");
+  if (!b->have_result) {
+    int rc = gsofa_batch_sync(b);
+    if (rc != GSOFA_OK) return rc;
+  }
+  fill_result(b->results[index], b->set.options(), b->preps[index].n, b->timing.last_ms, out);
+  return GSOFA_OK;
+}
+
+int gsofa_batch_nzu_get(gsofa_batch* b, int32_t index, int32_t* nzu) {
+  if (!b || !nzu) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!b->have_result) {
+    int rc = gsofa_batch_sync(b);
+    if (rc != GSOFA_OK) return rc;
+  }
+  return b->set.read_counts(index, nullptr, nullptr, nullptr, nzu);
+}
+
+int gsofa_batch_supernodes_get(gsofa_batch* b, int32_t index, uint8_t* flags) {
+  if (!b || !flags) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!b->have_result) {
+    int rc = gsofa_batch_sync(b);
+    if (rc != GSOFA_OK) return rc;
+  }
+  return b->set.read_counts(index, nullptr, nullptr, flags, nullptr);
+}
+
+void gsofa_batch_destroy(gsofa_batch* b) { delete b; }
+
+// ---- the reference's ASCII files ----------------------------------------------------------
+
+static int read_ints(const char* path, std::vector<int64_t>& out, int64_t expect_min) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  std::vector<char> buf;
+  char chunk[1 << 16];
+  size_t got;
+  while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+  std::fclose(f);
+  buf.push_back('\0');
+  out.clear();
+  char* p = buf.data();
+  while (true) {
+    while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') ++p;
+    if (!*p) break;
+    char* e = nullptr;
+    errno = 0;
+    const long long v = std::strtoll(p, &e, 10);
+    if (e == p || errno) return fail(GSOFA_ERR_IO, std::string("malformed integer in ") + path);
+    out.push_back(v);
+    p = e;
+  }
+  if ((int64_t)out.size() < expect_min) return fail(GSOFA_ERR_IO, std::string("too few integers in ") + path);
+  return GSOFA_OK;
+}
+
+int gsofa_read_ascii(const char* beg_file, const char* end_file, const char* csr_file, gsofa_host_matrix** out) {
+  if (!beg_file || !end_file || !csr_file || !out) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::vector<int64_t> c, b, e;
+  int rc = read_ints(csr_file, c, 2);
+  if (rc != GSOFA_OK) return rc;
+  const int64_t E = c[0], n = c[1];  // header order: edge count, then vertex count (...cuh:2577-2580)
+  if (E < 0 || n <= 0 || n > INT32_MAX || E > INT32_MAX) return fail(GSOFA_ERR_IO, "bad csr header");
+  if ((int64_t)c.size() < 2 + E) return fail(GSOFA_ERR_IO, "csr file holds fewer than E column ids");
+  rc = read_ints(beg_file, b, n);
+  if (rc != GSOFA_OK) return rc;
+  rc = read_ints(end_file, e, n);
+  if (rc != GSOFA_OK) return rc;
+  std::unique_ptr<gsofa_host_matrix> h(new (std::nothrow) gsofa_host_matrix);
+  if (!h) return fail(GSOFA_ERR_OUT_OF_MEMORY, "host matrix");
+  HostMatrix& M = h->m;
+  M.n = (int32_t)n;
+  M.adj.resize((size_t)E);
+  M.beg.resize((size_t)n);
+  M.end.resize((size_t)n);
+  for (int64_t k = 0; k < E; ++k) M.adj[k] = (int32_t)c[2 + k];
+  for (int64_t v = 0; v < n; ++v) {
+    M.beg[v] = (int32_t)b[v];
+    M.end[v] = (int32_t)e[v];
+  }
+  gsofa_csr view = M.view();
+  rc = validate_csr(&view);
+  if (rc != GSOFA_OK) return rc;
+  *out = h.release();
+  return GSOFA_OK;
+}
+
+int gsofa_write_ascii(const gsofa_csr* a, const char* beg_file, const char* end_file, const char* csr_file) {
+  int rc = validate_csr(a);
+  if (rc != GSOFA_OK) return rc;
+  FILE* f = std::fopen(csr_file, "wb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot create ") + csr_file);
+  std::fprintf(f, "%lld\n%d\n", (long long)a->nnz, a->n);
+  for (int64_t k = 0; k < a->nnz; ++k) std::fprintf(f, "%d ", a->adj[k]);
+  std::fclose(f);
+  f = std::fopen(beg_file, "wb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot create ") + beg_file);
+  for (int32_t v = 0; v < a->n; ++v) std::fprintf(f, "%d ", a->beg[v]);
+  std::fclose(f);
+  f = std::fopen(end_file, "wb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot create ") + end_file);
+  for (int32_t v = 0; v < a->n; ++v) std::fprintf(f, "%d ", a->end[v]);
+  std::fclose(f);
+  return GSOFA_OK;
+}
+
+// ---- the drop-in entry ------------------------------------------------------------------------
+
+int gsofa_symbfact_min_id(int argc, char** argv, int rank, uint64_t* fill_count, double* time_ms,
+                          int64_t* n_supernodes, uint64_t* edge_checks) {
+  if (argc != 11 || !argv) return fail(GSOFA_ERR_INVALID_ARG, "expected 10 arguments: beg end csr num_process "
+                                                              "chunk_size percent_cat0 percent_cat2 "
+                                                              "N_blocks_source_cat2 N_GPU_Node N_src_group");
+  if (!fill_count || !time_ms || !n_supernodes || !edge_checks) return fail(GSOFA_ERR_INVALID_ARG, "null output");
+  const int num_process = std::atoi(argv[4]);
+  const int chunk = std::atoi(argv[5]);
+  const int n_gpu_node = std::atoi(argv[9]);
+  if (num_process <= 0 || chunk <= 0 || n_gpu_node <= 0)
+    return fail(GSOFA_ERR_INVALID_ARG, "num_process, chunk size and N_GPU_Node must be positive");
+  gsofa_host_matrix* h = nullptr;
+  int rc = gsofa_read_ascii(argv[1], argv[2], argv[3], &h);
+  if (rc != GSOFA_OK) return rc;
+  std::unique_ptr<gsofa_host_matrix> hold(h);
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices <= 0) return fail(GSOFA_ERR_NO_DEVICE, "no HIP device");
+  gsofa_options o;
+  gsofa_options_default(&o);
+  o.chunk_size = chunk;
+  o.n_processes = num_process;
+  o.device = (rank % n_gpu_node) % devices;
+  gsofa_csr view = h->m.view();
+  gsofa_plan* plan = nullptr;
+  rc = gsofa_plan_create(&view, &o, &plan);
+  if (rc != GSOFA_OK) return rc;
+  std::unique_ptr<gsofa_plan, void (*)(gsofa_plan*)> hold_plan(plan, gsofa_plan_destroy);
+  rc = gsofa_run(plan, nullptr);
+  if (rc != GSOFA_OK) return rc;
+  gsofa_result r;
+  rc = gsofa_result_get(plan, &r);
+  if (rc != GSOFA_OK) return rc;
+  *fill_count += r.fill_count;
+  *time_ms += r.time_ms;
+  *n_supernodes = r.supernodes;
+  *edge_checks = r.edge_checks;
+  return GSOFA_OK;
+}
+
+}  // extern "C"

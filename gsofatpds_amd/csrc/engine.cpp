@@ -1,0 +1,350 @@
+// Host engine: input preparation, device state and the launch sequence of one set of matrices.
+#include "engine.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+
+namespace gsofa {
+
+int prepare(const gsofa_csr* a, Prepared& out) {
+  int rc = validate_csr(a);
+  if (rc != GSOFA_OK) return rc;
+  if (a->nnz > (int64_t)INT32_MAX - 64) return fail(GSOFA_ERR_CAPACITY, "nnz must stay below 2^31 - 64");
+  const int32_t n = a->n;
+  out.n = n;
+  out.rowptr.assign((size_t)n + 1, 0);
+  out.adj.clear();
+  out.adj.reserve((size_t)a->nnz);
+  out.dup_u.clear();
+  std::vector<int32_t> row;
+  bool any_dup = false;
+  std::vector<int32_t> dup((size_t)n, 0);
+  for (int32_t v = 0; v < n; ++v) {
+    row.assign(a->adj + a->beg[v], a->adj + a->end[v]);
+    std::sort(row.begin(), row.end());
+    int32_t upper_all = 0;
+    for (int32_t c : row) upper_all += c > v;
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    int32_t upper_distinct = 0;
+    for (int32_t c : row) upper_distinct += c > v;
+    dup[v] = upper_all - upper_distinct;
+    any_dup |= dup[v] != 0;
+    out.adj.insert(out.adj.end(), row.begin(), row.end());
+    out.rowptr[v + 1] = (int32_t)out.adj.size();
+  }
+  if (any_dup) out.dup_u.swap(dup);
+  return GSOFA_OK;
+}
+
+int device_info(int device, DeviceInfo& info) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(GSOFA_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= count) return fail(GSOFA_ERR_NO_DEVICE, "device ordinal out of range");
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipGetDeviceProperties");
+  info.cus = p.multiProcessorCount;
+  info.lds_per_block = p.sharedMemPerBlock;
+  info.lds_per_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : p.sharedMemPerBlock;
+  info.arch = p.gcnArchName;
+  if (info.arch.rfind("gfx950", 0) != 0)
+    return fail(GSOFA_ERR_NO_DEVICE, "engine is built for gfx950, device is " + info.arch);
+  return GSOFA_OK;
+}
+
+DeviceSet::~DeviceSet() {
+  for (void* p : owned_) (void)hipFree(p);
+}
+
+int DeviceSet::hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? GSOFA_ERR_OUT_OF_MEMORY : GSOFA_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+int DeviceSet::alloc(T*& p, size_t count, const char* what) {
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return hip_fail(e, what);
+  owned_.push_back(q);
+  p = static_cast<T*>(q);
+  return GSOFA_OK;
+}
+
+#define GSOFA_TRY(x)                 \
+  do {                               \
+    int rc_ = (x);                   \
+    if (rc_ != GSOFA_OK) return rc_; \
+  } while (0)
+#define GSOFA_HIP(x, what)                      \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt) {
+  opt_ = opt;
+  nmats_ = (int)mats.size();
+  if (nmats_ <= 0) return fail(GSOFA_ERR_INVALID_ARG, "empty matrix set");
+  if (nmats_ > (1 << 20)) return fail(GSOFA_ERR_CAPACITY, "too many matrices in one set");
+  if (opt.chunk <= 0) return fail(GSOFA_ERR_INVALID_ARG, "chunk size must be positive");
+  DeviceInfo info;
+  GSOFA_TRY(device_info(opt.device, info));
+  GSOFA_HIP(hipSetDevice(opt.device), "hipSetDevice");
+
+  ns_.resize(nmats_);
+  row_base_h_.assign(nmats_ + 1, 0);
+  nnz_base_h_.assign(nmats_ + 1, 0);
+  item_base_h_.assign(nmats_ + 1, 0);
+  blk_base_h_.assign(nmats_ + 1, 0);
+  n_max_ = 0;
+  bool any_dup = false;
+  for (int b = 0; b < nmats_; ++b) {
+    const Prepared& P = *mats[b];
+    ns_[b] = P.n;
+    n_max_ = std::max(n_max_, P.n);
+    row_base_h_[b + 1] = row_base_h_[b] + P.n;
+    nnz_base_h_[b + 1] = nnz_base_h_[b] + (int64_t)P.adj.size();
+    item_base_h_[b + 1] = item_base_h_[b] + 2 * (int64_t)P.n;
+    blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
+    any_dup |= !P.dup_u.empty();
+  }
+  nrows_ = row_base_h_[nmats_];
+  nnz_ = nnz_base_h_[nmats_];
+  nitems_ = item_base_h_[nmats_];
+  nblocks_sup_ = blk_base_h_[nmats_];
+  if (nnz_ > (int64_t)INT32_MAX - 64) return fail(GSOFA_ERR_CAPACITY, "total nnz of a set must stay below 2^31 - 64");
+
+  // ---- launch configuration ----
+  bw_max_ = (n_max_ + 31) / 32;
+  heavy_threads_ = opt.heavy_threads ? opt.heavy_threads : (n_max_ > 150000 ? 1024 : 512);
+  if (heavy_threads_ != 256 && heavy_threads_ != 512 && heavy_threads_ != 1024)
+    return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
+  const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : (heavy_threads_ == 1024 ? 1 : 2);
+  const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
+  const size_t bitmap_bytes = (size_t)bw_max_ * 4;
+  const size_t reserve = 512;  // static LDS of the kernel plus margin
+  if (bitmap_bytes + reserve + 256 * 8 > info.lds_per_block)
+    return fail(GSOFA_ERR_CAPACITY, "n = " + std::to_string(n_max_) + " exceeds the LDS bitmap capacity of the device");
+  if (opt.qcap) {
+    qcap_ = opt.qcap;
+  } else {
+    int64_t q = budget > bitmap_bytes + reserve ? (int64_t)(budget - bitmap_bytes - reserve) / 8 : 256;
+    q = std::max<int64_t>(256, std::min<int64_t>(16384, q));
+    qcap_ = (int32_t)(q & ~int64_t(63));
+  }
+  const size_t lds = heavy_lds_bytes(bw_max_, qcap_);
+  if (lds + reserve > info.lds_per_block) return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
+  GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds), "hipFuncSetAttribute(k_heavy)");
+  const int occ = heavy_occupancy(heavy_threads_, lds);
+  if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
+  heavy_grid_ = info.cus * occ;
+  const int occ_l = light_occupancy();
+  if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
+  light_grid_ = info.cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
+
+  // ---- allocations ----
+  GSOFA_TRY(alloc(rowptr_a_, nrows_ + 1, "rowptr A"));
+  GSOFA_TRY(alloc(adj_a_, nnz_, "adj A"));
+  GSOFA_TRY(alloc(ell_a_, nrows_ * kEll, "ell A"));
+  GSOFA_TRY(alloc(col_cnt_, nrows_, "column counts"));
+  GSOFA_TRY(alloc(rowptr_t_, nrows_ + 1, "rowptr A^T"));
+  GSOFA_TRY(alloc(cursor_, nrows_ + 1, "transpose cursor"));
+  GSOFA_TRY(alloc(adj_t_, nnz_, "adj A^T"));
+  GSOFA_TRY(alloc(ell_t_, nrows_ * kEll, "ell A^T"));
+  GSOFA_TRY(alloc(block_sums_, nrows_ / 4096 + 2, "scan block sums"));
+  GSOFA_TRY(alloc(total_, 1, "scan total"));
+  GSOFA_TRY(alloc(cnt_u_, nrows_, "U counts"));
+  GSOFA_TRY(alloc(cnt_l_, nrows_, "L counts"));
+  GSOFA_TRY(alloc(win_, nrows_ * 4, "window bits"));
+  GSOFA_TRY(alloc(sup_flag_, nrows_, "supernode flags"));
+  GSOFA_TRY(alloc(acc_, (size_t)nmats_ * kAccSlots, "accumulators"));
+  GSOFA_TRY(alloc(mats_d_, nmats_, "matrix table"));
+  GSOFA_TRY(alloc(item_base_d_, nmats_ + 1, "item base"));
+  GSOFA_TRY(alloc(row_base_d_, nmats_ + 1, "row base"));
+  GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
+  GSOFA_TRY(alloc(ws_, 1, "work state"));
+  GSOFA_TRY(alloc(heavy_items_, nitems_, "escalation list"));
+  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * 3 * (size_t)n_max_, "heavy spill regions"));
+  if (any_dup) GSOFA_TRY(alloc(dup_u_, nrows_, "duplicate surplus"));
+
+  // ---- uploads ----
+  std::vector<int32_t> rp((size_t)nrows_ + 1);
+  std::vector<int32_t> adj_all((size_t)std::max<int64_t>(nnz_, 1));
+  std::vector<int32_t> dup_all;
+  if (any_dup) dup_all.assign((size_t)nrows_, 0);
+  for (int b = 0; b < nmats_; ++b) {
+    const Prepared& P = *mats[b];
+    const int64_t rb = row_base_h_[b], nb = nnz_base_h_[b];
+    for (int32_t v = 0; v < P.n; ++v) rp[rb + v] = (int32_t)(nb + P.rowptr[v]);
+    std::copy(P.adj.begin(), P.adj.end(), adj_all.begin() + nb);
+    if (any_dup && !P.dup_u.empty()) std::copy(P.dup_u.begin(), P.dup_u.end(), dup_all.begin() + rb);
+  }
+  rp[nrows_] = (int32_t)nnz_;
+  GSOFA_HIP(hipMemcpy(rowptr_a_, rp.data(), rp.size() * 4, hipMemcpyHostToDevice), "upload rowptr");
+  if (nnz_ > 0) GSOFA_HIP(hipMemcpy(adj_a_, adj_all.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice), "upload adj");
+  if (any_dup) GSOFA_HIP(hipMemcpy(dup_u_, dup_all.data(), dup_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
+  GSOFA_HIP(hipMemcpy(item_base_d_, item_base_h_.data(), item_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
+  GSOFA_HIP(hipMemcpy(row_base_d_, row_base_h_.data(), row_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
+  GSOFA_HIP(hipMemcpy(blk_base_d_, blk_base_h_.data(), blk_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
+
+  mats_h_.assign(nmats_, MatDev{});
+  for (int b = 0; b < nmats_; ++b) {
+    MatDev& M = mats_h_[b];
+    const int64_t rb = row_base_h_[b];
+    M.n = ns_[b];
+    M.chunk = opt.chunk;
+    M.ell[0] = ell_a_ + rb * kEll;
+    M.ell[1] = ell_t_ + rb * kEll;
+    M.rowptr[0] = rowptr_a_ + rb;
+    M.rowptr[1] = rowptr_t_ + rb;
+    M.adj[0] = adj_a_;
+    M.adj[1] = adj_t_;
+    M.cnt[0] = cnt_u_ + rb;
+    M.cnt[1] = cnt_l_ + rb;
+    M.win = win_ + rb * 4;
+    M.dup_u = dup_u_ ? dup_u_ + rb : nullptr;
+    M.list_ptr[0] = M.list_ptr[1] = nullptr;
+    M.pool[0] = M.pool[1] = nullptr;
+    M.sup_flag = sup_flag_ + rb;
+    M.acc = acc_ + (size_t)b * kAccSlots;
+  }
+  GSOFA_HIP(hipMemcpy(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, hipMemcpyHostToDevice), "upload table");
+  return GSOFA_OK;
+}
+
+int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
+  // ev (optional, 5 events): before preparation, after preparation (ELL + transpose), after
+  // k_light, after k_heavy, after k_supernodes.
+  GSOFA_HIP(hipSetDevice(opt_.device), "hipSetDevice");
+  if (ev) GSOFA_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
+  GSOFA_HIP(launch_fill_u64(acc_, (int64_t)nmats_ * kAccSlots, 0ull, s), "clear accumulators");
+  GSOFA_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(ws_), 4, 0ull, s), "clear work state");
+  GSOFA_HIP(launch_ell_build(nrows_, rowptr_a_, adj_a_, ell_a_, s), "ell A");
+  GSOFA_HIP(launch_transpose(nrows_, nmats_, row_base_d_, rowptr_a_, adj_a_, col_cnt_, rowptr_t_, cursor_, adj_t_,
+                             block_sums_, total_, s),
+            "transpose");
+  GSOFA_HIP(launch_ell_build(nrows_, rowptr_t_, adj_t_, ell_t_, s), "ell A^T");
+  if (ev) GSOFA_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
+  GSOFA_HIP(launch_light(mats_d_, nmats_, item_base_d_, nitems_, ws_, heavy_items_, light_grid_, s), "k_light");
+  if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
+  GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, spill_, s),
+            "k_heavy");
+  if (ev) GSOFA_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
+  GSOFA_HIP(launch_supernodes(mats_d_, nmats_, blk_base_d_, nblocks_sup_, s), "k_supernodes");
+  if (ev) GSOFA_HIP(hipEventRecord(ev[4], s), "hipEventRecord");
+  return GSOFA_OK;
+}
+
+int DeviceSet::read_results(std::vector<MatResult>& out) {
+  std::vector<unsigned long long> acc((size_t)nmats_ * kAccSlots);
+  GSOFA_HIP(hipMemcpy(acc.data(), acc_, acc.size() * 8, hipMemcpyDeviceToHost), "read accumulators");
+  out.assign(nmats_, MatResult{});
+  for (int b = 0; b < nmats_; ++b) {
+    const unsigned long long* a = acc.data() + (size_t)b * kAccSlots;
+    if (a[kAccOverflow]) return fail(GSOFA_ERR_STATE, "structure offsets do not match this run's counts");
+    MatResult& r = out[b];
+    r.fill = a[kAccFill];
+    r.W = a[kAccW];
+    r.R = a[kAccR];
+    r.nnz_u = a[kAccNnzU];
+    r.nnz_l = a[kAccNnzL];
+    r.heavy = a[kAccHeavy];
+    r.heavy_w = a[kAccHeavyW];
+    r.heavy_r = a[kAccHeavyR];
+    r.heavy_nnz = a[kAccHeavyNnz];
+    r.nsup = (int64_t)a[kAccNsup];
+  }
+  return GSOFA_OK;
+}
+
+int DeviceSet::read_counts(int mat, int32_t* cnt_u, int32_t* cnt_l, uint8_t* sup_flag, int32_t* nzu) {
+  if (mat < 0 || mat >= nmats_) return fail(GSOFA_ERR_INVALID_ARG, "matrix index");
+  const int64_t rb = row_base_h_[mat];
+  const size_t n = (size_t)ns_[mat];
+  if (cnt_u || nzu) {
+    std::vector<int32_t> tmp(n);
+    GSOFA_HIP(hipMemcpy(tmp.data(), cnt_u_ + rb, n * 4, hipMemcpyDeviceToHost), "read U counts");
+    if (cnt_u) std::copy(tmp.begin(), tmp.end(), cnt_u);
+    if (nzu) {
+      std::vector<int32_t> dup(n, 0);
+      if (dup_u_) GSOFA_HIP(hipMemcpy(dup.data(), dup_u_ + rb, n * 4, hipMemcpyDeviceToHost), "read dup");
+      for (size_t i = 0; i < n; ++i) nzu[i] = tmp[i] + dup[i];
+    }
+  }
+  if (cnt_l) GSOFA_HIP(hipMemcpy(cnt_l, cnt_l_ + rb, n * 4, hipMemcpyDeviceToHost), "read L counts");
+  if (sup_flag) GSOFA_HIP(hipMemcpy(sup_flag, sup_flag_ + rb, n, hipMemcpyDeviceToHost), "read supernode flags");
+  return GSOFA_OK;
+}
+
+// Exact structure offsets from a completed run's counts: list_ptr = exclusive scan of counts.
+int DeviceSet::size_pools_from_counts() {
+  ptr_base_h_.assign(nmats_ + 1, 0);
+  for (int b = 0; b < nmats_; ++b) ptr_base_h_[b + 1] = ptr_base_h_[b] + ns_[b] + 1;
+  const int64_t nptr = ptr_base_h_[nmats_];
+  std::vector<int64_t> up((size_t)nptr), lp((size_t)nptr);
+  pool_base_u_.assign(nmats_ + 1, 0);
+  pool_base_l_.assign(nmats_ + 1, 0);
+  u_ptr_h_.assign(nmats_, {});
+  l_ptr_h_.assign(nmats_, {});
+  std::vector<int32_t> cu((size_t)nrows_), cl((size_t)nrows_);
+  GSOFA_HIP(hipMemcpy(cu.data(), cnt_u_, cu.size() * 4, hipMemcpyDeviceToHost), "read U counts");
+  GSOFA_HIP(hipMemcpy(cl.data(), cnt_l_, cl.size() * 4, hipMemcpyDeviceToHost), "read L counts");
+  for (int b = 0; b < nmats_; ++b) {
+    const int64_t rb = row_base_h_[b], pb = ptr_base_h_[b];
+    int64_t su = 0, sl = 0;
+    for (int32_t v = 0; v < ns_[b]; ++v) {
+      up[pb + v] = su;
+      lp[pb + v] = sl;
+      su += cu[rb + v];
+      sl += cl[rb + v];
+    }
+    up[pb + ns_[b]] = su;
+    lp[pb + ns_[b]] = sl;
+    pool_base_u_[b + 1] = pool_base_u_[b] + su;
+    pool_base_l_[b + 1] = pool_base_l_[b] + sl;
+    u_ptr_h_[b].assign(up.begin() + pb, up.begin() + pb + ns_[b] + 1);
+    l_ptr_h_[b].assign(lp.begin() + pb, lp.begin() + pb + ns_[b] + 1);
+  }
+  if (!list_ptr_u_) {
+    GSOFA_TRY(alloc(list_ptr_u_, (size_t)nptr, "U offsets"));
+    GSOFA_TRY(alloc(list_ptr_l_, (size_t)nptr, "L offsets"));
+    GSOFA_TRY(alloc(pool_u_, (size_t)pool_base_u_[nmats_], "U structure"));
+    GSOFA_TRY(alloc(pool_l_, (size_t)pool_base_l_[nmats_], "L structure"));
+  }
+  GSOFA_HIP(hipMemcpy(list_ptr_u_, up.data(), up.size() * 8, hipMemcpyHostToDevice), "upload U offsets");
+  GSOFA_HIP(hipMemcpy(list_ptr_l_, lp.data(), lp.size() * 8, hipMemcpyHostToDevice), "upload L offsets");
+  for (int b = 0; b < nmats_; ++b) {
+    MatDev& M = mats_h_[b];
+    M.list_ptr[0] = list_ptr_u_ + ptr_base_h_[b];
+    M.list_ptr[1] = list_ptr_l_ + ptr_base_h_[b];
+    M.pool[0] = pool_u_ + pool_base_u_[b];
+    M.pool[1] = pool_l_ + pool_base_l_[b];
+  }
+  GSOFA_HIP(hipMemcpy(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, hipMemcpyHostToDevice), "upload table");
+  pools_sized_ = true;
+  return GSOFA_OK;
+}
+
+int DeviceSet::read_structure(int mat, std::vector<int64_t>& u_ptr, std::vector<int32_t>& u_idx,
+                              std::vector<int64_t>& l_ptr, std::vector<int32_t>& l_idx) {
+  if (!pools_sized_) return fail(GSOFA_ERR_STATE, "structure was not produced (structure option off)");
+  if (mat < 0 || mat >= nmats_) return fail(GSOFA_ERR_INVALID_ARG, "matrix index");
+  u_ptr = u_ptr_h_[mat];
+  l_ptr = l_ptr_h_[mat];
+  const int64_t nu = u_ptr.back(), nl = l_ptr.back();
+  u_idx.resize((size_t)nu);
+  l_idx.resize((size_t)nl);
+  if (nu) GSOFA_HIP(hipMemcpy(u_idx.data(), pool_u_ + pool_base_u_[mat], (size_t)nu * 4, hipMemcpyDeviceToHost), "read U");
+  if (nl) GSOFA_HIP(hipMemcpy(l_idx.data(), pool_l_ + pool_base_l_[mat], (size_t)nl * 4, hipMemcpyDeviceToHost), "read L");
+  const int32_t n = ns_[mat];
+  for (int32_t v = 0; v < n; ++v) {
+    std::sort(u_idx.begin() + u_ptr[v], u_idx.begin() + u_ptr[v + 1]);
+    std::sort(l_idx.begin() + l_ptr[v], l_idx.begin() + l_ptr[v + 1]);
+  }
+  return GSOFA_OK;
+}
+
+}  // namespace gsofa

@@ -1,0 +1,116 @@
+// Host engine: prepared patterns, device state of a set of matrices, and the launch sequence.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gsofa_internal.h"
+#include "kernels.h"
+
+namespace gsofa {
+
+// A caller pattern after the reference's input step (row sort, ...cuh:2214-2227) plus
+// de-duplication: packed rowptr/adj with strictly increasing rows. dup_u[i] keeps the number of
+// duplicate entries > i that the reference would still count in its nzU (...cuh:896-901).
+struct Prepared {
+  int32_t n = 0;
+  std::vector<int32_t> rowptr;  // n + 1
+  std::vector<int32_t> adj;
+  std::vector<int32_t> dup_u;   // empty when the input had no duplicate upper entries
+};
+
+int prepare(const gsofa_csr* a, Prepared& out);
+
+struct Options {
+  int32_t chunk = 128;          // supernode chunk size S
+  int32_t n_processes = 1;      // reported supernodes = count + n_processes (...cuh:3549)
+  int32_t structure = 0;        // produce the explicit L and U structure
+  int32_t device = 0;
+  int32_t heavy_threads = 0;    // 0 = tuned
+  int32_t heavy_wgs_per_cu = 0; // 0 = tuned
+  int32_t light_wgs_per_cu = 0; // 0 = tuned
+  int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
+};
+
+struct MatResult {
+  uint64_t fill = 0, W = 0, R = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
+  uint64_t heavy_w = 0, heavy_r = 0, heavy_nnz = 0;  // the share of the workgroup path
+  int64_t nsup = 0;
+};
+
+struct DeviceInfo {
+  int cus = 0;
+  size_t lds_per_block = 0;
+  size_t lds_per_cu = 0;
+  std::string arch;
+};
+int device_info(int device, DeviceInfo& info);
+
+// Device state for one matrix (plain plan) or many (batch); all matrices share one launch of
+// each kernel.
+class DeviceSet {
+ public:
+  ~DeviceSet();
+  int create(const std::vector<const Prepared*>& mats, const Options& opt);
+  // Enqueue the whole symbolic factorisation on `s` (no host synchronisation, no allocation,
+  // no memset nodes: safe to capture in a graph).
+  int enqueue(hipStream_t s, hipEvent_t* stage_events = nullptr);
+  // After the stream completed: read per-matrix totals.
+  int read_results(std::vector<MatResult>& out);
+  int read_counts(int mat, int32_t* cnt_u, int32_t* cnt_l, uint8_t* sup_flag, int32_t* nzu);
+  // Structure of matrix `mat`: strict U rows (CSR) and strict L columns (CSC), sorted.
+  int read_structure(int mat, std::vector<int64_t>& u_ptr, std::vector<int32_t>& u_idx, std::vector<int64_t>& l_ptr,
+                     std::vector<int32_t>& l_idx);
+  bool structure_ready() const { return pools_sized_; }
+  int size_pools_from_counts();
+  int nmats() const { return nmats_; }
+  const std::vector<int32_t>& ns() const { return ns_; }
+  const Options& options() const { return opt_; }
+  int heavy_threads() const { return heavy_threads_; }
+  int heavy_grid() const { return heavy_grid_; }
+  int light_grid() const { return light_grid_; }
+  int qcap() const { return qcap_; }
+
+ private:
+  template <typename T>
+  int alloc(T*& p, size_t count, const char* what);
+  int hip_fail(hipError_t e, const char* what);
+
+  Options opt_;
+  int nmats_ = 0;
+  std::vector<int32_t> ns_;
+  std::vector<int64_t> row_base_h_, item_base_h_, blk_base_h_;
+  std::vector<int64_t> nnz_base_h_;
+  std::vector<int32_t> dup_any_;
+  int64_t nrows_ = 0, nnz_ = 0, nitems_ = 0, nblocks_sup_ = 0;
+  int32_t n_max_ = 0;
+  int heavy_threads_ = 1024, heavy_grid_ = 0, light_grid_ = 0;
+  int32_t qcap_ = 0, bw_max_ = 0;
+  bool pools_sized_ = false;
+
+  // device buffers (owned)
+  int32_t *rowptr_a_ = nullptr, *adj_a_ = nullptr, *ell_a_ = nullptr;
+  int32_t *col_cnt_ = nullptr, *rowptr_t_ = nullptr, *cursor_ = nullptr, *adj_t_ = nullptr, *ell_t_ = nullptr;
+  int32_t *block_sums_ = nullptr, *total_ = nullptr;
+  int32_t *cnt_u_ = nullptr, *cnt_l_ = nullptr, *dup_u_ = nullptr;
+  uint32_t* win_ = nullptr;
+  uint8_t* sup_flag_ = nullptr;
+  unsigned long long* acc_ = nullptr;
+  MatDev* mats_d_ = nullptr;
+  int64_t *item_base_d_ = nullptr, *row_base_d_ = nullptr, *blk_base_d_ = nullptr;
+  WorkState* ws_ = nullptr;
+  uint64_t* heavy_items_ = nullptr;
+  int32_t* spill_ = nullptr;
+  int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries
+  int32_t *pool_u_ = nullptr, *pool_l_ = nullptr;
+  std::vector<int64_t> ptr_base_h_;                        // matrix b's offsets start at row_base + b
+  std::vector<int64_t> pool_base_u_, pool_base_l_;        // per matrix, into the pools
+  std::vector<std::vector<int64_t>> u_ptr_h_, l_ptr_h_;    // host copies of the offsets
+  std::vector<MatDev> mats_h_;
+  std::vector<void*> owned_;
+};
+
+}  // namespace gsofa

@@ -1,0 +1,90 @@
+// Device-side data layout shared by the HIP kernels and the host engine.
+#pragma once
+
+#include <stdint.h>
+
+namespace gsofa {
+
+// Rows are packed into 64-byte slots of 16 int32 ("ELL-16"): the entries first, then -1
+// padding. A row with more than 16 entries keeps its first 15 in the slot and stores
+// -(p + 2) in entry 15, where p is the CSR position of its 16th entry; the rest of the row is
+// read from the CSR adjacency up to rowptr[v + 1].
+constexpr int kEll = 16;
+constexpr int32_t kPad = -1;
+
+// Accumulator slots (u64) per matrix.
+enum : int {
+  kAccFill = 0,      // off-diagonal fill-ins, both triangles
+  kAccW = 1,         // schedule-independent edge checks of the row traversals
+  kAccR = 2,         // sum of |R_i| of the row traversals
+  kAccOverflow = 5,  // nonzero: a row's count differs from the structure offsets
+  kAccNsup = 6,      // supernodes counted by the reference rule
+  kAccNnzU = 7,      // strict upper entries of F
+  kAccNnzL = 8,      // strict lower entries of F
+  kAccHeavy = 9,     // sources that took the workgroup path
+  kAccHeavyW = 10,   // edge checks of row traversals done on the workgroup path
+  kAccHeavyR = 11,   // |R_i| of row traversals done on the workgroup path
+  kAccHeavyNnz = 12, // strict F entries found on the workgroup path (both directions)
+  kAccSlots = 16
+};
+
+// One matrix as the kernels see it. Direction 0 traverses A (rows, gives U), direction 1
+// traverses A^T (columns of A, gives L by columns).
+struct MatDev {
+  int32_t n;
+  int32_t chunk;                  // supernode chunk size S
+  const int32_t* ell[2];          // n * kEll
+  const int32_t* rowptr[2];       // n + 1 (overflow rows only)
+  const int32_t* adj[2];          // CSR adjacency (overflow rows only)
+  int32_t* cnt[2];                // n: |U_i| (dir 0) / |L column j| (dir 1)
+  uint32_t* win;                  // n * 4: bits t of column j: row j+1+t in L, same chunk
+  const int32_t* dup_u;           // n: duplicate surplus of A's upper entries (may be null)
+  const int64_t* list_ptr[2];     // n + 1: row/column offsets into pool (null: no structure)
+  int32_t* pool[2];               // strict U rows (dir 0) / strict L columns (dir 1)
+  uint8_t* sup_flag;              // n: 1 where the supernode count increments
+  unsigned long long* acc;        // kAccSlots
+};
+
+// Scheduler state of one launch set (plain matrix or batch).
+struct WorkState {
+  unsigned long long light_next;   // next item for the wave path
+  unsigned long long heavy_count;  // items escalated to the workgroup path
+  unsigned long long heavy_next;   // next escalated item
+  unsigned long long pad;
+};
+
+// Item encoding for the escalation list: matrix index, direction, source row.
+__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t src) {
+  return ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)src;
+}
+
+}  // namespace gsofa
+
+// ---- launchers (defined in kernels.hip) ---------------------------------------------------
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+namespace gsofa {
+hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s);
+hipError_t launch_ell_build(int64_t nrows, const int32_t* rowptr, const int32_t* adj, int32_t* ell, hipStream_t s);
+// Builds A^T of every matrix: col_cnt/cursor/rowptr_t have nrows (+1) entries, block_sums
+// ceil(nrows / 4096), total one int.
+hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
+                            const int32_t* adj, int32_t* col_cnt, int32_t* rowptr_t, int32_t* cursor,
+                            int32_t* adj_t, int32_t* block_sums, int32_t* total, hipStream_t s);
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws,
+                        uint64_t* heavy_items, int grid, hipStream_t s);
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap);
+hipError_t heavy_set_lds_limit(int threads, size_t bytes);
+hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s);
+hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
+constexpr int kLightThreads = 256;
+}  // namespace gsofa
+
+namespace gsofa {
+// Resident workgroups of k_heavy per CU for a block size and dynamic LDS size.
+int heavy_occupancy(int threads, size_t lds_bytes);
+int light_occupancy();
+}  // namespace gsofa

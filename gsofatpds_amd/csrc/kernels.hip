@@ -1,0 +1,826 @@
+// HIP kernels of the symbolic-factorisation engine, written for gfx950 (CDNA4, wave64).
+//
+// Method. The reference decides row i of F = L+U by a per-source minimax fill-path traversal
+// (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:777-1029): (i,m) is in F iff
+// m is in row i of A or a path i -> x1 -> ... -> xk -> m exists in G(A) with every
+// xj < min(i,m) (Rose-Tarjan). This engine evaluates the same predicate as two plain
+// reachability traversals per index:
+//   * direction 0, on A from source i through vertices < i: every m > i adjacent to the
+//     reached set (or to i) is in U row i; the reached set below i is R_i;
+//   * direction 1, on A^T from source j through vertices < j: every m > j adjacent to the
+//     reached set (or to j) is in L column j, since the reversed path has all
+//     intermediates below j = min(m, j).
+// A reachability traversal needs one bit of state per vertex instead of a 32-bit minimax
+// label, so a whole traversal of a 1.2M-vertex pattern fits in one CU's LDS.
+//
+// Kernels, in launch order for one matrix or one batch:
+//   k_fill_u64 / k_fill_i32     clear accumulators and counters (graph-safe, no memset nodes)
+//   k_ell_build                 pack rows into 64-byte ELL-16 slots
+//   k_col_hist, scan, k_transpose_scatter, k_ell_build   build A^T in the same layout
+//   k_light                     one wave per source, LDS hash set; escalates big sources
+//   k_heavy                     one workgroup per escalated source, n-bit LDS bitmap
+//   k_supernodes                the reference's supernode rule, one thread per row
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace gsofa {
+
+// ------------------------------------------------------------------------------------------
+// wave64 helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Pointers that reach a kernel through a struct are generic; these casts make the hot accesses
+// global_load / global_store instead of flat ones.
+#define GSOFA_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const GSOFA_GLOBAL T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  *(GSOFA_GLOBAL T*)(p) = v;
+}
+typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4i32 gload_v4(const int32_t* p) { return *(const GSOFA_GLOBAL v4i32*)(p); }
+
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor(lo, o, 64);
+    hi = __shfl_xor(hi, o, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int find_segment(const int64_t* base, int nseg, int64_t k) {
+  // base[0] = 0 <= ... <= base[nseg]; returns b with base[b] <= k < base[b + 1]
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------------------------------
+// small utility kernels (graph-capturable replacements for memset)
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long* p, int64_t count, unsigned long long v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// layout: ELL-16 packing and the transpose
+// ------------------------------------------------------------------------------------------
+// Row g of the concatenated pattern: entries adj[rowptr[g] .. rowptr[g+1]) (local column ids).
+__global__ void k_ell_build(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ adj,
+                            int32_t* __restrict__ ell) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = rowptr[g], e = rowptr[g + 1];
+    const int32_t deg = e - b;
+    int32_t slot[kEll];
+#pragma unroll
+    for (int k = 0; k < kEll; ++k) slot[k] = kPad;
+    if (deg <= kEll) {
+      for (int k = 0; k < deg; ++k) slot[k] = adj[b + k];
+    } else {
+      for (int k = 0; k < kEll - 1; ++k) slot[k] = adj[b + k];
+      slot[kEll - 1] = -(b + (kEll - 1) + 2);
+    }
+    int4* dst = reinterpret_cast<int4*>(ell + g * kEll);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_int4(slot[4 * q], slot[4 * q + 1], slot[4 * q + 2], slot[4 * q + 3]);
+  }
+}
+
+// Column counts of every matrix of the set: one thread per global row.
+__global__ void k_col_hist(int64_t nrows, int nmats, const int64_t* __restrict__ row_base,
+                           const int32_t* __restrict__ rowptr, const int32_t* __restrict__ adj,
+                           int32_t* __restrict__ col_cnt) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (int64_t)gridDim.x * blockDim.x) {
+    const int mb = find_segment(row_base, nmats, g);
+    const int64_t base = row_base[mb];
+    for (int32_t p = rowptr[g]; p < rowptr[g + 1]; ++p) atomicAdd(&col_cnt[base + adj[p]], 1);
+  }
+}
+
+// cursor[] starts as the exclusive scan of col_cnt (global positions); rows scatter themselves
+// into their columns. The order inside a column is not deterministic and does not matter:
+// every output is a set or a count.
+__global__ void k_transpose_scatter(int64_t nrows, int nmats, const int64_t* __restrict__ row_base,
+                                    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ adj,
+                                    int32_t* __restrict__ cursor, int32_t* __restrict__ adj_t) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (int64_t)gridDim.x * blockDim.x) {
+    const int mb = find_segment(row_base, nmats, g);
+    const int64_t base = row_base[mb];
+    const int32_t v = (int32_t)(g - base);
+    for (int32_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
+      const int32_t pos = atomicAdd(&cursor[base + adj[p]], 1);
+      adj_t[pos] = v;
+    }
+  }
+}
+
+// Exclusive scan of int32 counts into int32 positions (three passes; counts fit in int32).
+constexpr int kScanBlock = 1024;
+constexpr int kScanItems = 4;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* __restrict__ in, int64_t count,
+                                                            int32_t* __restrict__ block_sums) {
+  __shared__ int32_t s[kScanBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems;
+  int32_t acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
+    if (i < count) acc += in[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane_id() == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t t = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) t += s[w];
+    block_sums[blockIdx.x] = t;
+  }
+}
+
+// Single block: exclusive scan of the block sums in place; writes the grand total to *total.
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(int32_t* __restrict__ sums, int64_t nblocks,
+                                                            int32_t* __restrict__ total) {
+  __shared__ int32_t s_wave[kScanBlock / 64];
+  __shared__ int32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nblocks; base += kScanBlock) {
+    const int64_t i = base + threadIdx.x;
+    const int32_t v = (i < nblocks) ? sums[i] : 0;
+    int32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      if (lane_id() >= o) x += y;
+    }
+    if (lane_id() == 63) s_wave[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int32_t wave_off = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wave_off += s_wave[w];
+    const int32_t carry = s_carry;
+    if (i < nblocks) sums[i] = carry + wave_off + x - v;
+    __syncthreads();
+    if (threadIdx.x == kScanBlock - 1) s_carry = carry + wave_off + x;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = s_carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int32_t* __restrict__ in, int64_t count,
+                                                           const int32_t* __restrict__ block_off,
+                                                           int32_t* __restrict__ out, int32_t* __restrict__ out2,
+                                                           const int32_t* __restrict__ total) {
+  __shared__ int32_t s_wave[kScanBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems;
+  int32_t carry = block_off[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
+    const int32_t v = (i < count) ? in[i] : 0;
+    int32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      if (lane_id() >= o) x += y;
+    }
+    if (lane_id() == 63) s_wave[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int32_t wave_off = 0, all = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+      if (w < (int)(threadIdx.x >> 6)) wave_off += s_wave[w];
+      all += s_wave[w];
+    }
+    if (i < count) {
+      out[i] = carry + wave_off + x - v;
+      if (out2) out2[i] = carry + wave_off + x - v;
+    }
+    carry += all;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[count] = *total;
+    if (out2) out2[count] = *total;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// traversal: shared pieces
+// ------------------------------------------------------------------------------------------
+struct RowCursor {
+  int32_t e[kEll];
+  int32_t ov_pos, ov_end;  // overflow part of a long row, read from the CSR adjacency
+};
+
+__device__ __forceinline__ void load_row(const MatDev& M, int dir, int32_t v, RowCursor& rc) {
+  rc.ov_pos = 0;
+  rc.ov_end = 0;
+  if (v < 0) {
+#pragma unroll
+    for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
+    return;
+  }
+  const int32_t* src = M.ell[dir] + (int64_t)v * kEll;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const v4i32 t = gload_v4(src + 4 * q);
+    rc.e[4 * q] = t.x;
+    rc.e[4 * q + 1] = t.y;
+    rc.e[4 * q + 2] = t.z;
+    rc.e[4 * q + 3] = t.w;
+  }
+  if (rc.e[kEll - 1] <= -2) {
+    rc.ov_pos = -(rc.e[kEll - 1] + 2);
+    rc.ov_end = gload(M.rowptr[dir] + v + 1);
+    rc.e[kEll - 1] = kPad;
+  }
+}
+
+// Window bits of direction 1: column j = src, row m > j, same supernode chunk, m - j - 1 < 128.
+// Selects instead of win[t >> 5]: a dynamically indexed register array would live in scratch.
+__device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t m, int32_t chunk_end) {
+  const int32_t t = m - src - 1;
+  if (t < 128 && m < chunk_end) {
+    const uint32_t b = 1u << (t & 31);
+    const int w = t >> 5;
+    win[0] |= (w == 0) ? b : 0u;
+    win[1] |= (w == 1) ? b : 0u;
+    win[2] |= (w == 2) ? b : 0u;
+    win[3] |= (w == 3) ? b : 0u;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// k_light: one wave per source, LDS hash set of the reached vertices
+// ------------------------------------------------------------------------------------------
+constexpr int kLightWaves = 4;
+constexpr int kHashBits = 10;
+constexpr int kHash = 1 << kHashBits;
+constexpr int kLightLimit = 512;             // reached vertices before escalation
+constexpr int kLightQ = kLightLimit + 64;    // R entries never exceed the limit plus one row step
+
+__device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) {
+  uint32_t h = ((uint32_t)m * 0x9E3779B1u) >> (32 - kHashBits);
+  // The table is at most (kLightLimit + 64) / kHash full, so an empty slot is always found.
+  while (true) {
+    const int32_t old = atomicCAS(&H[h], -1, m);
+    if (old == -1) return 1;
+    if (old == m) return 0;
+    h = (h + 1) & (kHash - 1);
+  }
+}
+
+__global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __restrict__ mats, int nmats,
+                                                            const int64_t* __restrict__ item_base, int64_t nitems,
+                                                            WorkState* __restrict__ ws,
+                                                            uint64_t* __restrict__ heavy_items) {
+  __shared__ int32_t s_hash[kLightWaves][kHash];
+  __shared__ int32_t s_q[kLightWaves][kLightQ];
+  const int wave = threadIdx.x >> 6;
+  const int lane = lane_id();
+  int32_t* H = s_hash[wave];
+  int32_t* Q = s_q[wave];
+  for (int i = lane; i < kHash; i += 64) H[i] = -1;
+
+  // per-wave running totals, flushed once at exit (one atomic per wave and slot)
+  uint64_t t_fill = 0, t_w = 0, t_r = 0, t_nnz_u = 0, t_nnz_l = 0;
+  int last_mat = -1;
+
+  const int64_t gwave = (int64_t)blockIdx.x * kLightWaves + wave;
+  const int64_t nwaves = (int64_t)gridDim.x * kLightWaves;
+
+  auto flush = [&](int mb) {
+    if (mb < 0) return;
+    const uint64_t f = wave_sum_u64(t_fill), w = wave_sum_u64(t_w), r = wave_sum_u64(t_r);
+    const uint64_t nu = wave_sum_u64(t_nnz_u), nl = wave_sum_u64(t_nnz_l);
+    if (lane == 0) {
+      unsigned long long* acc = mats[mb].acc;
+      if (f) atomicAdd(&acc[kAccFill], f);
+      if (w) atomicAdd(&acc[kAccW], w);
+      if (r) atomicAdd(&acc[kAccR], r);
+      if (nu) atomicAdd(&acc[kAccNnzU], nu);
+      if (nl) atomicAdd(&acc[kAccNnzL], nl);
+    }
+    t_fill = t_w = t_r = t_nnz_u = t_nnz_l = 0;
+  };
+
+  for (int64_t k = gwave; k < nitems; k += nwaves) {
+    const int mb = (nmats == 1) ? 0 : find_segment(item_base, nmats, k);
+    if (mb != last_mat) {
+      flush(last_mat);
+      last_mat = mb;
+    }
+    const MatDev& M = mats[mb];
+    const int64_t local = k - item_base[mb];
+    const int dir = (int)(local & 1);
+    const int32_t src = M.n - 1 - (int32_t)(local >> 1);
+    const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
+
+    if (lane == 0) H[((uint32_t)src * 0x9E3779B1u) >> (32 - kHashBits)] = src;  // empty table
+    int qn = 0, hcount = 1;
+    bool esc = false;
+    uint32_t u_cnt = 0, u_seed = 0;
+    uint64_t w_acc = 0;
+    uint32_t win[4] = {0, 0, 0, 0};
+
+    // batch 0 is the seed row (lane 0 holds src); later batches take up to 64 queued vertices
+    int head = 0;
+    bool seed = true;
+    while (true) {
+      int32_t v;
+      int batch_end;
+      if (seed) {
+        v = (lane == 0) ? src : -1;
+        batch_end = 0;
+      } else {
+        batch_end = min(head + 64, qn);
+        v = (head + lane < batch_end) ? Q[head + lane] : -1;
+      }
+      RowCursor rc;
+      load_row(M, dir, v, rc);
+      uint32_t deg = 0;
+      auto visit = [&](int32_t m) {
+        const bool active = m >= 0;
+        deg += active;
+        const int ins = active ? hash_insert(H, m) : 0;
+        const bool is_r = ins == 1 && m < src;
+        const bool is_u = ins == 1 && m > src;
+        const uint64_t mr = __ballot(is_r);
+        const uint64_t mn = __ballot(ins == 1);
+        if (is_r) {
+          const int pos = qn + (int)rank_below(mr);
+          if (pos < kLightQ) Q[pos] = m;
+        }
+        qn += __popcll(mr);
+        hcount += __popcll(mn);
+        if (is_u) {
+          ++u_cnt;
+          if (seed) ++u_seed;
+          if (dir == 1) window_bit(win, src, m, chunk_end);
+        }
+        if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+      };
+      // ELL part (entries first, padding last), then the CSR tail of rows longer than 16
+#pragma unroll
+      for (int j = 0; j < kEll; ++j) {
+        if (!esc && __ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+      }
+      while (!esc && __ballot(rc.ov_pos < rc.ov_end) != 0) {
+        int32_t m = kPad;
+        if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
+        visit(m);
+      }
+      w_acc += deg;
+      if (esc) break;
+      if (seed) {
+        seed = false;
+      } else {
+        head = batch_end;
+      }
+      if (head >= qn) break;
+    }
+
+    if (esc) {
+      if (lane == 0) {
+        const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
+        heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)src);
+      }
+      for (int i = lane; i < kHash; i += 64) H[i] = -1;
+      continue;
+    }
+
+    const uint32_t U = (uint32_t)wave_sum_u64(u_cnt);
+    const uint32_t Useed = (uint32_t)wave_sum_u64(u_seed);
+    const uint64_t W = wave_sum_u64(w_acc);
+    if (lane == 0) {
+      M.cnt[dir][src] = (int32_t)U;
+      t_fill += U - Useed;
+      if (dir == 0) {
+        t_w += W;
+        t_r += (uint64_t)qn;
+        t_nnz_u += U;
+      } else {
+        t_nnz_l += U;
+      }
+    }
+    if (dir == 1) {
+      const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
+      const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
+      if (lane == 0) reinterpret_cast<uint4*>(M.win)[src] = make_uint4(w0, w1, w2, w3);
+    }
+    // structure: the U entries (m > src) are the hash members above src
+    if (M.list_ptr[dir] != nullptr) {
+      const int64_t off = gload(M.list_ptr[dir] + src);
+      const int64_t end = gload(M.list_ptr[dir] + src + 1);
+      if (end - off != (int64_t)U) {
+        if (lane == 0) atomicOr(&M.acc[kAccOverflow], 1ull);
+      } else {
+        int64_t wpos = off;
+        for (int i = lane; i < kHash; i += 64) {
+          const int32_t m = H[i];
+          const bool take = m > src;
+          const uint64_t mt = __ballot(take);
+          if (take) gstore(M.pool[dir] + wpos + rank_below(mt), m);
+          wpos += __popcll(mt);
+        }
+      }
+    }
+    for (int i = lane; i < kHash; i += 64) H[i] = -1;
+  }
+  flush(last_mat);
+}
+
+// ------------------------------------------------------------------------------------------
+// k_heavy: one workgroup per escalated source, n-bit bitmap in LDS, level-synchronous
+// ------------------------------------------------------------------------------------------
+// Dynamic LDS: bitmap[bw_max] | q0[qcap] | q1[qcap]. Queue indices >= qcap spill to the
+// workgroup's global region: spill0 / spill1 (n_max entries each), ustage (n_max entries).
+__device__ __forceinline__ int32_t qget(const int32_t* l, const int32_t* g, int32_t qcap, int32_t e) {
+  return e < qcap ? l[e] : gload(g + (e - qcap));
+}
+__device__ __forceinline__ void qput(int32_t* l, int32_t* g, int32_t qcap, int32_t e, int32_t v) {
+  if (e < qcap) l[e] = v;
+  else gstore(g + (e - qcap), v);
+}
+
+struct Totals {
+  uint64_t fill = 0, w = 0, r = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
+};
+// Every k_heavy total is also the workgroup path's own share: W, R and F entries found there.
+
+__device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals& t) {
+  if (mb < 0) return;
+  unsigned long long* acc = mats[mb].acc;
+  if (t.fill) atomicAdd(&acc[kAccFill], t.fill);
+  if (t.w) atomicAdd(&acc[kAccW], t.w);
+  if (t.r) atomicAdd(&acc[kAccR], t.r);
+  if (t.nnz_u) atomicAdd(&acc[kAccNnzU], t.nnz_u);
+  if (t.nnz_l) atomicAdd(&acc[kAccNnzL], t.nnz_l);
+  if (t.heavy) atomicAdd(&acc[kAccHeavy], t.heavy);
+  if (t.w) atomicAdd(&acc[kAccHeavyW], t.w);
+  if (t.r) atomicAdd(&acc[kAccHeavyR], t.r);
+  if (t.nnz_u + t.nnz_l) atomicAdd(&acc[kAccHeavyNnz], t.nnz_u + t.nnz_l);
+  t = Totals{};
+}
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ mats,
+                                                    WorkState* __restrict__ ws,
+                                                    const uint64_t* __restrict__ heavy_items,
+                                                    int32_t bw_max, int32_t qcap, int64_t n_max,
+                                                    int32_t* __restrict__ spill_all) {
+  extern __shared__ uint32_t dyn[];
+  __shared__ unsigned long long s_idx;
+  __shared__ int32_t s_cnt[3];
+  __shared__ uint32_t s_u, s_useed, s_ust;
+  __shared__ uint32_t s_win[4];
+  __shared__ unsigned long long s_w;
+  __shared__ int64_t s_off;
+
+  constexpr int kWaves = kThreads / 64;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  uint32_t* bm = dyn;
+  int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
+  int32_t* const ql1 = ql0 + qcap;
+  int32_t* const region = spill_all + (int64_t)blockIdx.x * 3 * n_max;
+  int32_t* const qg0 = region;
+  int32_t* const qg1 = region + n_max;
+  int32_t* const ustage = region + 2 * n_max;
+  (void)kWaves;
+
+  // thread 0 keeps per-matrix totals for the matrix it last worked on
+  int last_mat = -1;
+  Totals tot;
+
+  const unsigned long long total = ws->heavy_count;
+  while (true) {
+    if (tid == 0) s_idx = atomicAdd(&ws->heavy_next, 1ull);
+    __syncthreads();
+    const unsigned long long idx = s_idx;
+    if (idx >= total) break;
+    const uint64_t item = heavy_items[idx];
+    const int mb = (int)(item >> 33);
+    const int dir = (int)((item >> 32) & 1);
+    const int32_t src = (int32_t)(item & 0xffffffffu);
+    const MatDev& M = mats[mb];
+    const int32_t n = M.n;
+    const int32_t bw = (n + 31) >> 5;
+    const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
+    const bool want_list = M.list_ptr[dir] != nullptr;
+
+    for (int32_t w = tid; w < bw; w += kThreads) bm[w] = 0u;
+    if (tid == 0) {
+      s_cnt[0] = 1;
+      s_cnt[1] = 0;
+      s_cnt[2] = 0;
+      s_u = s_useed = s_ust = 0;
+      s_win[0] = s_win[1] = s_win[2] = s_win[3] = 0;
+      s_w = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      bm[src >> 5] |= 1u << (src & 31);
+      ql0[0] = src;
+    }
+    __syncthreads();
+
+    uint64_t w_acc = 0;
+    uint32_t u_cnt = 0, u_seed = 0;
+    uint32_t win[4] = {0, 0, 0, 0};
+    int64_t r_total = 0;
+    for (int level = 0;; ++level) {
+      const int32_t ncur = s_cnt[level % 3];
+      if (ncur == 0) break;
+      if (level > 0) r_total += ncur;
+      if (tid == 0) s_cnt[(level + 2) % 3] = 0;
+      const bool odd = (level & 1) != 0;
+      const int32_t* const cur_l = odd ? ql1 : ql0;
+      const int32_t* const cur_g = odd ? qg1 : qg0;
+      int32_t* const nxt_l = odd ? ql0 : ql1;
+      int32_t* const nxt_g = odd ? qg0 : qg1;
+      int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
+      const bool seed = level == 0;
+      for (int32_t base = 0; base < ncur; base += kThreads) {
+        const int32_t e = base + tid;
+        const int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+        RowCursor rc;
+        load_row(M, dir, v, rc);
+        uint32_t deg = 0;
+        auto visit = [&](int32_t m) {
+          const bool active = m >= 0;
+          deg += active;
+          bool isnew = false;
+          if (active) {
+            const uint32_t bit = 1u << (m & 31);
+            uint32_t* wp = &bm[m >> 5];
+            if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+          }
+          const bool is_r = isnew && m < src;
+          const bool is_u = isnew && m > src;
+          const uint64_t mr = __ballot(is_r);
+          if (mr) {
+            const int leader = __ffsll((unsigned long long)mr) - 1;
+            int32_t basepos = 0;
+            if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
+            basepos = __shfl(basepos, leader, 64);
+            if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
+          }
+          if (is_u) {
+            ++u_cnt;
+            if (seed) ++u_seed;
+            if (dir == 1) window_bit(win, src, m, chunk_end);
+          }
+          if (want_list) {
+            const uint64_t mu = __ballot(is_u);
+            if (mu) {
+              const int leader = __ffsll((unsigned long long)mu) - 1;
+              uint32_t basepos = 0;
+              if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
+              basepos = __shfl(basepos, leader, 64);
+              if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+            }
+          }
+        };
+#pragma unroll
+        for (int j = 0; j < kEll; ++j) {
+          if (__ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+        }
+        while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+          int32_t m = kPad;
+          if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
+          visit(m);
+        }
+        w_acc += deg;
+      }
+      __syncthreads();
+    }
+
+    // reduce the per-thread counters of this source
+    {
+      const uint64_t uc = wave_sum_u64(u_cnt), us = wave_sum_u64(u_seed), wa = wave_sum_u64(w_acc);
+      const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
+      const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
+      if (lane == 0) {
+        if (uc) atomicAdd(&s_u, (uint32_t)uc);
+        if (us) atomicAdd(&s_useed, (uint32_t)us);
+        if (wa) atomicAdd(&s_w, (unsigned long long)wa);
+        if (w0) atomicOr(&s_win[0], w0);
+        if (w1) atomicOr(&s_win[1], w1);
+        if (w2) atomicOr(&s_win[2], w2);
+        if (w3) atomicOr(&s_win[3], w3);
+      }
+    }
+    __syncthreads();
+    const uint32_t U = s_u;
+    if (tid == 0) {
+      if (mb != last_mat) {
+        flush_totals(mats, last_mat, tot);
+        last_mat = mb;
+      }
+      M.cnt[dir][src] = (int32_t)U;
+      tot.fill += U - s_useed;
+      tot.heavy += 1;
+      if (dir == 0) {
+        tot.w += s_w;
+        tot.r += (uint64_t)r_total;
+        tot.nnz_u += U;
+      } else {
+        tot.nnz_l += U;
+        reinterpret_cast<uint4*>(M.win)[src] = make_uint4(s_win[0], s_win[1], s_win[2], s_win[3]);
+      }
+      if (want_list) {
+        int64_t off = gload(M.list_ptr[dir] + src);
+        if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U) {
+          atomicOr(&M.acc[kAccOverflow], 1ull);
+          off = -1;
+        }
+        s_off = off;
+      }
+    }
+    if (want_list) {
+      __syncthreads();
+      const int64_t off = s_off;
+      if (off >= 0)
+        for (uint32_t i = tid; i < U; i += kThreads) gstore(M.pool[dir] + off + i, gload(ustage + i));
+    }
+    __syncthreads();
+  }
+  if (tid == 0) flush_totals(mats, last_mat, tot);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// k_supernodes: the reference's rule (warpcentric_local_detection_test,
+// gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:1790-2123), one thread per row.
+//   start(r): (r mod S) mod 128 == 0, or nzU[r-1] - nzU[r] != 1        (:1833, :1879, :1928)
+//   other rows pass through; a pass-through row r extends the run of the last start s of its
+//   chunk iff (r, s) is in F (:2034), else it counts one more supernode (:2047).
+// Starts recur at least every 128 rows of a chunk, so s >= r - 127 and (r, s) is one of the
+// 128 window bits of column s.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int32_t nzu_at(const MatDev& M, int32_t r) {
+  return M.cnt[0][r] + (M.dup_u ? M.dup_u[r] : 0);
+}
+
+__device__ __forceinline__ bool is_start(const MatDev& M, int32_t r) {
+  const int32_t off = r % M.chunk;
+  if ((off & 127) == 0) return true;
+  return nzu_at(M, r - 1) - nzu_at(M, r) != 1;
+}
+
+__global__ __launch_bounds__(256) void k_supernodes(const MatDev* __restrict__ mats, int nmats,
+                                                    const int64_t* __restrict__ blk_base) {
+  // blk_base[b] = first block of matrix b; each block covers 256 rows of one matrix
+  __shared__ uint32_t s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  const int mb = find_segment(blk_base, nmats, (int64_t)blockIdx.x);
+  const MatDev& M = mats[mb];
+  const int64_t r64 = ((int64_t)blockIdx.x - blk_base[mb]) * 256 + threadIdx.x;
+  uint32_t flag = 0;
+  if (r64 < M.n) {
+    const int32_t r = (int32_t)r64;
+    if (is_start(M, r)) {
+      flag = 1;
+    } else {
+      int32_t s = r - 1;
+      while (!is_start(M, s)) --s;
+      const int32_t t = r - s - 1;
+      const uint32_t word = M.win[(int64_t)s * 4 + (t >> 5)];
+      flag = ((word >> (t & 31)) & 1u) ? 0u : 1u;
+    }
+    M.sup_flag[r] = (uint8_t)flag;
+  }
+  const uint64_t c = wave_sum_u64(flag);
+  if (lane_id() == 0 && c) atomicAdd(&s_sum, (uint32_t)c);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum) atomicAdd(&M.acc[kAccNsup], (unsigned long long)s_sum);
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side launchers (the engine calls these; kernels stay private to this file)
+// ------------------------------------------------------------------------------------------
+static int grid_for(int64_t work, int block, int cap) {
+  int64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(count, 256, 4096)), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, p, count, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_ell_build(int64_t nrows, const int32_t* rowptr, const int32_t* adj, int32_t* ell, hipStream_t s) {
+  hipLaunchKernelGGL(k_ell_build, dim3(grid_for(nrows, 256, 16384)), dim3(256), 0, s, nrows, rowptr, adj, ell);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
+                            const int32_t* adj, int32_t* col_cnt, int32_t* rowptr_t, int32_t* cursor,
+                            int32_t* adj_t, int32_t* block_sums, int32_t* total, hipStream_t s) {
+  const int g = grid_for(nrows, 256, 16384);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nrows, 256, 8192)), dim3(256), 0, s, col_cnt, nrows, 0);
+  hipLaunchKernelGGL(k_col_hist, dim3(g), dim3(256), 0, s, nrows, nmats, row_base, rowptr, adj, col_cnt);
+  const int64_t per = (int64_t)kScanBlock * kScanItems;
+  const int64_t nb = (nrows + per - 1) / per;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, col_cnt, nrows, block_sums);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, s, block_sums, nb, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanBlock), 0, s, col_cnt, nrows, block_sums, rowptr_t,
+                     cursor, total);
+  hipLaunchKernelGGL(k_transpose_scatter, dim3(g), dim3(256), 0, s, nrows, nmats, row_base, rowptr, adj, cursor, adj_t);
+  return hipGetLastError();
+}
+
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws,
+                        uint64_t* heavy_items, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_light, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems, ws, heavy_items);
+  return hipGetLastError();
+}
+
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap) { return (size_t)bw_max * 4 + (size_t)qcap * 8; }
+
+hipError_t heavy_set_lds_limit(int threads, size_t bytes) {
+  switch (threads) {
+    case 1024: return hipFuncSetAttribute((const void*)k_heavy<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    case 512: return hipFuncSetAttribute((const void*)k_heavy<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    case 256: return hipFuncSetAttribute((const void*)k_heavy<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s) {
+  const size_t lds = heavy_lds_bytes(bw_max, qcap);
+  switch (threads) {
+    case 1024:
+      hipLaunchKernelGGL(k_heavy<1024>, dim3(grid), dim3(1024), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      break;
+    case 512:
+      hipLaunchKernelGGL(k_heavy<512>, dim3(grid), dim3(512), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      break;
+    case 256:
+      hipLaunchKernelGGL(k_heavy<256>, dim3(grid), dim3(256), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_supernodes, dim3((unsigned)nblocks), dim3(256), 0, s, mats, nmats, blk_base);
+  return hipGetLastError();
+}
+
+}  // namespace gsofa
+
+namespace gsofa {
+int heavy_occupancy(int threads, size_t lds_bytes) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (threads) {
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<1024>, 1024, lds_bytes); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<512>, 512, lds_bytes); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<256>, 256, lds_bytes); break;
+    default: break;
+  }
+  return e == hipSuccess ? nb : 0;
+}
+
+int light_occupancy() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_light, kLightWaves * 64, 0) != hipSuccess) return 0;
+  return nb;
+}
+}  // namespace gsofa

@@ -1,0 +1,166 @@
+/*
+ * gsofa.h - C ABI of the MI355X symbolic-factorisation engine (libgsofa.so).
+ *
+ * Plain C types only: int32 CSR in the reference's layout (row v is adj[beg[v]..end[v]),
+ * rows may be non-contiguous and unsorted), caller-owned inputs, library-owned plans.
+ * Every entry point returns GSOFA_OK (0) or a GSOFA_ERR_* code and never exits the process;
+ * gsofa_last_error() gives the text of the calling thread's last failure.
+ */
+#ifndef GSOFA_H
+#define GSOFA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define GSOFA_OK 0
+#define GSOFA_ERR_INVALID_ARG 1   /* bad argument count / value                                */
+#define GSOFA_ERR_IO 2            /* a file could not be opened or parsed                      */
+#define GSOFA_ERR_BAD_CSR 3       /* beg > end, extent outside adj, or column id outside [0,n) */
+#define GSOFA_ERR_HIP 4           /* a HIP runtime call failed                                 */
+#define GSOFA_ERR_NO_DEVICE 5     /* no usable gfx950 device                                   */
+#define GSOFA_ERR_OUT_OF_MEMORY 6 /* device or host allocation failed                          */
+#define GSOFA_ERR_CAPACITY 7      /* a request exceeds a fixed capacity                        */
+#define GSOFA_ERR_STATE 8         /* call out of order (e.g. result before run)                */
+#define GSOFA_ERR_REJECTED 9      /* serving: request can never fit the device budget          */
+#define GSOFA_ERR_CANCELLED 10    /* serving: request cancelled before completion              */
+#define GSOFA_ERR_NOT_READY 11    /* serving: result not ready yet                             */
+
+/* ---- the pattern --------------------------------------------------------------------------- */
+typedef struct gsofa_csr {
+  int32_t n;          /* rows = columns                                                     */
+  int64_t nnz;        /* length of adj                                                      */
+  const int32_t* beg; /* n: first entry of row v                                            */
+  const int32_t* end; /* n: one past the last entry of row v                                */
+  const int32_t* adj; /* nnz column ids                                                     */
+} gsofa_csr;
+
+const char* gsofa_last_error(void);
+
+/* ---- host patterns: generators and the reference's ASCII files -------------------------- */
+typedef struct gsofa_host_matrix gsofa_host_matrix;
+
+/* generator kinds / orderings / drop sides (see gsofa_generate) */
+#define GSOFA_GEN_STENCIL 0
+#define GSOFA_GEN_ERDOS_RENYI 1
+#define GSOFA_ORDER_NATURAL 0
+#define GSOFA_ORDER_ND 1
+#define GSOFA_ORDER_RCM 2
+#define GSOFA_DROP_ANY 0
+#define GSOFA_DROP_LOWER 1
+#define GSOFA_DROP_UPPER 2
+
+/* the workloads: member selects a batch member / stream request (ignored otherwise) */
+#define GSOFA_WL_GRID2D_4096 1   /* 64x64 5-point, natural order, 20% drop                 */
+#define GSOFA_WL_GRID3D_110592 2 /* 48^3 7-point, ND, 15% upper drop                        */
+#define GSOFA_WL_GRID2D_659033 3 /* 812-wide 9-point truncated to 659,033 rows, ND, extras */
+#define GSOFA_WL_BATCH256 4      /* member k of the 256-matrix batch                        */
+#define GSOFA_WL_STREAM 5        /* request k of the served stream                          */
+
+int gsofa_generate_workload(int workload, int64_t member, gsofa_host_matrix** out);
+
+/* Free-form generator. params (int64 each): kind, dims, points, nx, ny, nz, n_keep, order,
+ * nd_leaf, drop_side, drop_ppm, extra_every, avg_degree_x100. */
+#define GSOFA_GEN_NPARAMS 13
+int gsofa_generate(const int64_t* params, uint64_t seed, gsofa_host_matrix** out);
+
+int gsofa_host_matrix_csr(const gsofa_host_matrix* m, gsofa_csr* view);
+void gsofa_host_matrix_free(gsofa_host_matrix* m);
+
+/* Reference ASCII triplet (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:2567-2616):
+ * csr file = "E n" then E column ids; beg file = n ints; end file = n ints. Rows may be
+ * non-contiguous and unsorted. Unlike the reference, failures return codes instead of exit(1). */
+int gsofa_read_ascii(const char* beg_file, const char* end_file, const char* csr_file, gsofa_host_matrix** out);
+int gsofa_write_ascii(const gsofa_csr* a, const char* beg_file, const char* end_file, const char* csr_file);
+
+/* ---- the drop-in entry ---------------------------------------------------------------------
+ * Replaces void symbfact_min_id(int args, char** argv, int myrank, ull_t& fill_count,
+ * double& time, int& N_supernodes, ull_t& Process_TEPS)
+ * (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:2529; caller
+ * gSoFa_SOAComparison/main.cu:41). Same argv contract (argc == 11): [1] beg file, [2] end file,
+ * [3] csr file, [4] num_process, [5] max_supernode_size (chunk S), [6] percent_cat0,
+ * [7] percent_cat2, [8] N_blocks_source_cat2, [9] N_GPU_Node, [10] N_src_group.
+ * All num_process x N_GPU_Node shares run on one device (device = rank % N_GPU_Node).
+ * Outputs as in the reference: *fill_count += fill-ins, *time_ms += milliseconds from the first
+ * kernel to the end of supernode detection, *n_supernodes = supernodes counted + num_process
+ * (each reference process adds one, ...cuh:3549, summed by main.cu:44), *edge_checks =
+ * schedule-independent edge checks W. */
+int gsofa_symbfact_min_id(int argc, char** argv, int rank, uint64_t* fill_count, double* time_ms,
+                          int64_t* n_supernodes, uint64_t* edge_checks);
+
+/* Device facts the engine sizes itself by: compute units, LDS bytes per workgroup and per CU,
+ * and the architecture name (e.g. "gfx950:sramecc+:xnack-"). No counterpart in the reference. */
+int gsofa_device_query(int device, int32_t* cus, int64_t* lds_per_block, int64_t* lds_per_cu, char* arch,
+                       int32_t arch_len);
+
+/* ---- structured API --------------------------------------------------------------------- */
+typedef struct gsofa_options {
+  int32_t chunk_size;       /* supernode chunk S (argv[5]); default 128                      */
+  int32_t n_processes;      /* argv[4]: the reported supernode count adds one per process      */
+  int32_t structure;        /* 1: also produce strict L (by columns) and strict U (by rows)    */
+  int32_t device;           /* HIP device ordinal                                               */
+  int32_t heavy_threads;    /* tuning: workgroup size of the bitmap path, 0 = automatic         */
+  int32_t heavy_wgs_per_cu; /* tuning: resident bitmap workgroups per CU, 0 = automatic         */
+  int32_t light_wgs_per_cu; /* tuning: resident hash-path workgroups per CU, 0 = automatic      */
+  int32_t queue_capacity;   /* tuning: LDS frontier entries per level buffer, 0 = automatic     */
+} gsofa_options;
+
+typedef struct gsofa_result {
+  uint64_t fill_count;        /* off-diagonal fill-ins: |F \ A| ("Number of fill-ins detected") */
+  uint64_t nnz_F;             /* |F| = |L+U| including the diagonal                           */
+  uint64_t nnz_L, nnz_U;      /* strict lower / strict upper entries of F                     */
+  uint64_t edge_checks;       /* W = sum_i deg(i) + sum_{v in R_i} deg(v)                      */
+  uint64_t reach;             /* sum_i |R_i|                                                  */
+  int64_t supernodes;         /* reported count: counted + n_processes                       */
+  int64_t supernodes_counted; /* supernodes counted by the reference rule                     */
+  uint64_t heavy_sources;     /* traversals that took the workgroup (bitmap) path              */
+  double time_ms;             /* device time of the last run                                   */
+} gsofa_result;
+
+typedef struct gsofa_plan gsofa_plan;
+
+void gsofa_options_default(gsofa_options* opt);
+/* Validates and copies the pattern (row sort + de-duplication, the reference's SortEdgeList),
+ * allocates device state and uploads it. The caller keeps ownership of a. */
+int gsofa_plan_create(const gsofa_csr* a, const gsofa_options* opt, gsofa_plan** out);
+/* Enqueues one full symbolic factorisation on `hip_stream` (a hipStream_t, 0 = default
+ * stream). With the structure option the first run also performs a sizing pass. */
+int gsofa_run(gsofa_plan* plan, void* hip_stream);
+int gsofa_sync(gsofa_plan* plan);
+int gsofa_result_get(gsofa_plan* plan, gsofa_result* out);
+/* nzU[i] as the reference counts it (U entries of row i, duplicates of A included); n ints. */
+int gsofa_nzu_get(gsofa_plan* plan, int32_t* nzu);
+/* flags[i] = 1 where the supernode count increments (a start or a failed pass-through); n bytes. */
+int gsofa_supernodes_get(gsofa_plan* plan, uint8_t* flags);
+/* Sizes of the strict parts; requires the structure option and a completed run. */
+int gsofa_structure_size(gsofa_plan* plan, int64_t* nnz_u, int64_t* nnz_l);
+/* Strict U by rows (u_rowptr n+1, u_col nnz_u) and strict L by columns (l_colptr n+1,
+ * l_row nnz_l), each row/column sorted ascending. */
+int gsofa_structure_get(gsofa_plan* plan, int64_t* u_rowptr, int32_t* u_col, int64_t* l_colptr, int32_t* l_row);
+/* Device time of the last run's stages, from events on the run's stream: ms[0] preparation
+ * (ELL packing + transpose), ms[1] the wave (hash) path, ms[2] the workgroup (bitmap) path,
+ * ms[3] supernode detection. No counterpart in the reference (its only timers are wtime()). */
+int gsofa_stage_times(gsofa_plan* plan, double* ms);
+/* The workgroup path's share of the last run: row-traversal edge checks, |R_i| sum, and strict
+ * F entries (both directions). Used to price that kernel alone. */
+int gsofa_work_split(gsofa_plan* plan, uint64_t* heavy_edge_checks, uint64_t* heavy_reach, uint64_t* heavy_entries);
+void gsofa_plan_destroy(gsofa_plan* plan);
+
+/* ---- batched form: many matrices in one captured graph of persistent launches ----------- */
+typedef struct gsofa_batch gsofa_batch;
+int gsofa_batch_create(const gsofa_csr* mats, int32_t count, const gsofa_options* opt, gsofa_batch** out);
+/* Replays the batch's hipGraph on `hip_stream` (captured on the first call). */
+int gsofa_batch_run(gsofa_batch* batch, void* hip_stream);
+int gsofa_batch_sync(gsofa_batch* batch);
+int gsofa_batch_result_get(gsofa_batch* batch, int32_t index, gsofa_result* out);
+int gsofa_batch_nzu_get(gsofa_batch* batch, int32_t index, int32_t* nzu);
+int gsofa_batch_supernodes_get(gsofa_batch* batch, int32_t index, uint8_t* flags);
+void gsofa_batch_destroy(gsofa_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSOFA_H */

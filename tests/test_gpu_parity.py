@@ -1,0 +1,196 @@
+"""GPU parity: the HIP engine against the oracle, bit-exact, through the C ABI."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import gsofatpds_amd as g
+import oracle
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def load_ds():
+    z = np.load(os.path.join(GOLDEN, "ds_11948.npz"))
+    return g.Pattern(int(len(z["beg"])), z["beg"], z["end"], z["adj"])
+
+
+def check(p, S=128, structure=True, label="", **opt):
+    say(f"[parity] {label} n={p.n} nnz={p.nnz} S={S} structure={structure} opt={opt}")
+    o = oracle.symbfact(p, chunk=S, structure=structure)
+    with g.Plan(p, chunk_size=S, structure=int(structure), **opt) as plan:
+        plan.run()
+        r = plan.result()
+        assert r.fill_count == o["fill"]
+        assert r.nnz_F == o["nnz_F"]
+        assert r.edge_checks == o["W"]
+        assert r.reach == o["R"]
+        assert r.supernodes_counted == o["nsup"]
+        assert r.supernodes == o["nsup"] + 1
+        assert np.array_equal(plan.nzu(), o["nzu"])
+        assert np.array_equal(plan.supernode_flags(), o["sup_flag"])
+        if structure:
+            up, ui, lp, li = plan.structure()
+            ou, oui, ol, oli = oracle.split_structure(p.n, o["F_rowptr"], o["F_cols"])
+            assert np.array_equal(up, ou) and np.array_equal(ui, oui)
+            assert np.array_equal(lp, ol) and np.array_equal(li, oli)
+    return r
+
+
+def test_device_is_gfx950():
+    info = g.device_query(0)
+    say(f"[device] {info}")
+    assert info["arch"].startswith("gfx950")
+
+
+def test_smallest_patterns():
+    check(g.Pattern.from_dense(np.ones((1, 1), bool)), label="1x1")
+    check(g.Pattern.from_dense(np.eye(2, dtype=bool)), label="2x2 diag")
+    check(g.Pattern.from_dense(np.ones((3, 3), bool)), label="3x3 dense")
+
+
+def test_grid_4096():
+    for S in (128, 64, 256):
+        check(g.generate_workload(g.WL_GRID2D_4096), S=S, label="grid2d_4096")
+
+
+def test_reference_dataset():
+    r = check(load_ds(), S=128, label="ds_11948")
+    assert r.fill_count == 980136
+    check(load_ds(), S=1024, structure=False, label="ds_11948")
+
+
+@pytest.mark.parametrize("n,density,seed", [(7, 0.3, 1), (100, 0.05, 2), (257, 0.02, 3), (1000, 0.004, 4),
+                                            (3000, 0.0015, 5), (3000, 0.003, 6)])
+def test_random_unsymmetric(n, density, seed):
+    rng = np.random.default_rng(seed)
+    D = rng.random((n, n)) < density
+    np.fill_diagonal(D, rng.random(n) < 0.9)  # some diagonals missing
+    D[rng.integers(0, n, 3)] = False          # some empty rows
+    check(g.Pattern.from_dense(D), label=f"random({n},{density})")
+
+
+def test_long_rows_take_the_csr_tail():
+    rng = np.random.default_rng(9)
+    n = 2000
+    D = rng.random((n, n)) < 0.002
+    np.fill_diagonal(D, True)
+    D[rng.integers(0, n, 20)] |= rng.random((20, n)) < 0.05  # rows of ~100 entries
+    D[:, 1500] |= rng.random(n) < 0.1                         # a long column (long A^T row)
+    check(g.Pattern.from_dense(D), label="long rows")
+
+
+def test_duplicates_and_unsorted_rows():
+    """Duplicate column ids are counted in nzU as the reference does; F and fills are sets."""
+    rng = np.random.default_rng(4)
+    n = 600
+    D = rng.random((n, n)) < 0.01
+    np.fill_diagonal(D, True)
+    p = g.Pattern.from_dense(D)
+    rows = []
+    for v in range(n):
+        r = list(p.adj[p.beg[v]:p.end[v]])
+        if v % 7 == 0 and r:
+            r += r[-2:]  # duplicates, including upper entries
+        rng.shuffle(r)
+        rows.append(np.array(r, np.int32))
+    ptr = np.zeros(n + 1, np.int64)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    q = g.Pattern.from_rowptr(ptr, np.concatenate(rows))
+    check(q, label="duplicates")
+
+
+def test_noncontiguous_rows():
+    """Rows given by beg/end in arbitrary order inside adj, as in the reference's dataset."""
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    order = np.random.default_rng(1).permutation(p.n)
+    adj, beg, end, pos = [], np.zeros(p.n, np.int32), np.zeros(p.n, np.int32), 0
+    for v in order:
+        r = p.adj[p.beg[v]:p.end[v]]
+        beg[v], end[v] = pos, pos + len(r)
+        adj.append(r)
+        pos += len(r)
+    check(g.Pattern(p.n, beg, end, np.concatenate(adj)), label="noncontiguous")
+
+
+@pytest.mark.parametrize("member", [6, 7, 11])
+def test_batch_members(member):
+    p = g.generate_workload(g.WL_BATCH256, member)
+    check(p, structure=p.n < 3000, label=f"batch member {member}")
+
+
+def test_tuning_configurations_are_identical():
+    p = load_ds()
+    base = None
+    for opt in (dict(), dict(heavy_threads=256), dict(heavy_threads=1024), dict(queue_capacity=256),
+                dict(light_wgs_per_cu=1, heavy_wgs_per_cu=1)):
+        say(f"[tuning] {opt}")
+        with g.Plan(p, **opt) as plan:
+            plan.run()
+            r = plan.result()
+            got = (r.fill_count, r.nnz_F, r.edge_checks, r.reach, r.supernodes, plan.nzu().tobytes(),
+                   plan.supernode_flags().tobytes())
+        if base is None:
+            base = got
+        assert got == base
+
+
+def test_repeated_runs_and_stream():
+    import torch
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    s = torch.cuda.Stream()
+    with g.Plan(p, structure=1) as plan:
+        plan.run(s).sync()
+        first = plan.result().as_dict()
+        st = plan.structure()
+        for _ in range(3):
+            plan.run(s).sync()
+            r = plan.result().as_dict()
+            first["time_ms"] = r["time_ms"]
+            assert r == first
+            st2 = plan.structure()
+            assert all(np.array_equal(a, b) for a, b in zip(st, st2))
+
+
+def test_batch_equals_plain():
+    members = [6, 7, 11, 8, 2]
+    pats = [g.generate_workload(g.WL_BATCH256, m) for m in members]
+    say(f"[batch] members {members} n={[p.n for p in pats]}")
+    b = g.Batch(pats)
+    for rep in range(2):
+        b.run().sync()
+        for i, p in enumerate(pats):
+            with g.Plan(p) as plan:
+                plan.run()
+                r = plan.result()
+                rb = b.result(i)
+                for k in ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes"):
+                    assert getattr(rb, k) == getattr(r, k), (rep, i, k)
+                assert np.array_equal(b.nzu(i), plan.nzu())
+                assert np.array_equal(b.supernode_flags(i), plan.supernode_flags())
+    b.close()
+
+
+def test_entry_and_cli_on_reference_files(tmp_path):
+    ds = load_ds()
+    files = [str(tmp_path / f) for f in ("colbeg_11948.dat", "colend_11948.dat", "csr_11948.dat")]
+    g.write_ascii(ds, *files)
+    argv = ["gsofa", *files, "1", "128", "0", "0", "1", "1", "4096"]
+    fill, ms, nsup, teps = g.symbfact_min_id(argv)
+    o = oracle.symbfact(ds, chunk=128)
+    assert fill == 980136 and nsup == o["nsup"] + 1 and teps == o["W"] and ms > 0
+    r = subprocess.run([g.CLI_PATH, *argv[1:]], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Final Number of fill-ins detected: 980136" in r.stdout
+    assert f"Final Number of Supernodes detected: {o['nsup'] + 1}" in r.stdout
+    csv = (tmp_path / "Vary_percent_blocksi_pre2.csv").read_text().splitlines()
+    assert csv[0] == "vert_count;N_gpus;p0;p1;#blocks_src;min_time;max_time"
+    assert csv[1].startswith(files[1] + ";1;0;0;0;")

@@ -1,0 +1,164 @@
+"""Host-side checks without a GPU: the C ABI library loads and exports what include/gsofa.h
+declares, generators, the reference's ASCII format, input validation and error codes."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gsofatpds_amd as g
+from conftest import ROOT, gpu_available
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "gsofa.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(gsofa_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = g.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_header_compiles_as_c():
+    src = os.path.join(ROOT, "include", "gsofa.h")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_workload_sizes():
+    p1 = g.generate_workload(g.WL_GRID2D_4096)
+    assert p1.n == 4096 and 16000 < p1.nnz < 18000
+    p2 = g.generate_workload(g.WL_GRID3D_110592)
+    assert p2.n == 110592 and 650_000 < p2.nnz < 760_000
+
+
+@pytest.mark.slow
+def test_workload_659033():
+    p3 = g.generate_workload(g.WL_GRID2D_659033)
+    assert p3.n == 659033 and 5_500_000 < p3.nnz < 6_000_000
+
+
+def test_generators_deterministic_and_well_formed():
+    for wl, m in ((g.WL_GRID2D_4096, 0), (g.WL_BATCH256, 6), (g.WL_BATCH256, 7), (g.WL_STREAM, 3)):
+        a = g.generate_workload(wl, m)
+        b = g.generate_workload(wl, m)
+        assert np.array_equal(a.adj, b.adj) and np.array_equal(a.beg, b.beg)
+        for v in range(a.n):
+            row = a.adj[a.beg[v]:a.end[v]]
+            assert v in row  # diagonal always present
+            assert np.all(np.diff(row) > 0)  # sorted, no duplicates
+            assert row.min() >= 0 and row.max() < a.n
+
+
+def test_batch_member_sizes():
+    ns = [g.generate_workload(g.WL_BATCH256, k).n for k in range(0, 256, 17)]
+    assert all(2000 <= n <= 200000 for n in ns)
+
+
+def test_nd_ordering_is_a_permutation_with_separators_last():
+    # 2-D 5-point 20x20 ND: the last vertex belongs to the top separator, a full grid line
+    p = g.generate(kind=g.GEN_STENCIL, dims=2, points=5, nx=20, ny=20, order=g.ORDER_ND, nd_leaf=16)
+    assert p.n == 400
+    rows = {v: set(p.adj[p.beg[v]:p.end[v]]) for v in range(p.n)}
+    # symmetric pattern (no drops)
+    assert all(v in rows[u] for v in rows for u in rows[v])
+    # the top separator (20 vertices, ids 380..399) splits the rest into two halves with no edges
+    sep = set(range(380, 400))
+    rest = [v for v in range(p.n) if v not in sep]
+    comp = {}
+    for s in rest:
+        if s in comp:
+            continue
+        stack = [s]
+        comp[s] = s
+        while stack:
+            v = stack.pop()
+            for u in rows[v]:
+                if u not in sep and u not in comp:
+                    comp[u] = s
+                    stack.append(u)
+    assert len(set(comp.values())) == 2
+
+
+def test_ascii_round_trip(tmp_path):
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    files = [str(tmp_path / f) for f in ("beg.dat", "end.dat", "csr.dat")]
+    g.write_ascii(p, *files)
+    q = g.read_ascii(*files)
+    assert q.n == p.n and np.array_equal(q.adj, p.adj) and np.array_equal(q.beg, p.beg)
+
+
+def test_ascii_reads_reference_dataset_if_present():
+    d = "/root/reference/SpaceOptimization/dataset"
+    if not os.path.isdir(d):
+        pytest.skip("reference dataset not mounted")
+    p = g.read_ascii(f"{d}/colbeg_11948.dat", f"{d}/colend_11948.dat", f"{d}/csr_11948.dat")
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ds_11948.npz"))
+    assert p.n == 11948 and np.array_equal(p.adj, z["adj"]) and np.array_equal(p.beg, z["beg"])
+
+
+def test_ascii_errors(tmp_path):
+    with pytest.raises(g.GsofaError) as e:
+        g.read_ascii(str(tmp_path / "a"), str(tmp_path / "b"), str(tmp_path / "c"))
+    assert e.value.code == g.ERR_IO
+    (tmp_path / "c").write_text("5 3\n0 1 2 0 9\n")
+    (tmp_path / "a").write_text("0 2 4\n")
+    (tmp_path / "b").write_text("2 4 5\n")
+    with pytest.raises(g.GsofaError) as e:
+        g.read_ascii(str(tmp_path / "a"), str(tmp_path / "b"), str(tmp_path / "c"))
+    assert e.value.code == g.ERR_BAD_CSR  # column id 9 >= n = 3
+
+
+def bad_patterns():
+    ok = g.Pattern.from_dense(np.eye(4, dtype=bool))
+    col = g.Pattern(4, ok.beg.copy(), ok.end.copy(), ok.adj.copy())
+    col.adj[2] = 7
+    ext = g.Pattern(4, ok.beg.copy(), ok.end.copy(), ok.adj.copy())
+    ext.beg[1], ext.end[1] = 3, 2
+    past = g.Pattern(4, ok.beg.copy(), ok.end.copy(), ok.adj.copy())
+    past.end[3] = 9
+    neg = g.Pattern(4, ok.beg.copy(), ok.end.copy(), ok.adj.copy())
+    neg.adj[0] = -1
+    return [col, ext, past, neg]
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_malformed_csr_is_rejected_before_any_device_work(idx):
+    p = bad_patterns()[idx]
+    with pytest.raises(g.GsofaError) as e:
+        g.Plan(p)
+    assert e.value.code == g.ERR_BAD_CSR
+
+
+def test_entry_argument_count():
+    with pytest.raises(g.GsofaError) as e:
+        g.symbfact_min_id(["x", "a", "b"])
+    assert e.value.code == g.ERR_INVALID_ARG
+
+
+def test_entry_missing_file(tmp_path):
+    argv = ["gsofa", str(tmp_path / "b"), str(tmp_path / "e"), str(tmp_path / "c"), "1", "128", "0", "0", "1", "1",
+            "4096"]
+    with pytest.raises(g.GsofaError) as e:
+        g.symbfact_min_id(argv)
+    assert e.value.code == g.ERR_IO
+
+
+def test_no_device_is_an_error_code_not_a_fallback():
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(g.GsofaError) as e:
+        g.Plan(g.Pattern.from_dense(np.eye(8, dtype=bool)))
+    assert e.value.code == g.ERR_NO_DEVICE
+
+
+def test_cli_usage():
+    r = subprocess.run([g.CLI_PATH], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
