@@ -74,7 +74,6 @@ def main():
               f"supernode chunks cyclic over devices, one 32-byte reduction) is designed but not built; "
               f"--gpus {args.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
-    import torch
     import gsofatpds_amd as g
 
     wl = {"grid2d_659033": g.WL_GRID2D_659033, "grid3d_110592": g.WL_GRID3D_110592,
@@ -90,7 +89,7 @@ def main():
     for i in range(args.warmup):
         plan.run().sync()
         log(f"[bench] warmup {i}: {plan.result().time_ms:.3f} ms")
-    torch.cuda.synchronize()
+    g.device_synchronize()
     stage_acc = {"prep_ms": 0.0, "light_ms": 0.0, "heavy_ms": 0.0, "supernodes_ms": 0.0}
     event_ms = 0.0
     t0 = time.perf_counter()
@@ -100,7 +99,7 @@ def main():
         for k in stage_acc:
             stage_acc[k] += st[k]
         event_ms += plan.result().time_ms
-    torch.cuda.synchronize()
+    g.device_synchronize()
     wall = time.perf_counter() - t0
     r = plan.result()
     split = plan.work_split()

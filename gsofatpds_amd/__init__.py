@@ -62,7 +62,7 @@ class CSR(C.Structure):
 class Options(C.Structure):
     _fields_ = [("chunk_size", C.c_int32), ("n_processes", C.c_int32), ("structure", C.c_int32),
                 ("device", C.c_int32), ("heavy_threads", C.c_int32), ("heavy_wgs_per_cu", C.c_int32),
-                ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32)]
+                ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32), ("chain_block", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -89,6 +89,9 @@ _SIGNATURES = {
                                         C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
     "gsofa_device_query": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.c_char_p, C.c_int32]),
+    "gsofa_device_synchronize": (C.c_int, [C.c_int]),
+    "gsofa_stream_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "gsofa_stream_destroy": (C.c_int, [C.c_void_p]),
     "gsofa_options_default": (None, [C.POINTER(Options)]),
     "gsofa_plan_create": (C.c_int, [C.POINTER(CSR), C.POINTER(Options), C.POINTER(C.c_void_p)]),
     "gsofa_run": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -237,6 +240,36 @@ def device_query(device: int = 0) -> dict:
     return {"cus": cus.value, "lds_per_block": lb.value, "lds_per_cu": lc.value, "arch": arch.value.decode()}
 
 
+def device_synchronize(device: int = 0) -> None:
+    _check(lib().gsofa_device_synchronize(device))
+
+
+class Stream:
+    """A HIP stream owned by the engine's runtime (use instead of a framework's stream: one
+    process must not run two HIP runtimes)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().gsofa_stream_create(device, C.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().gsofa_stream_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+
+def gpu_present() -> bool:
+    """True if the engine's HIP runtime sees a device (does not touch any other runtime)."""
+    try:
+        device_query(0)
+        return True
+    except (GsofaError, RuntimeError):
+        return False
+
+
 def default_options(**kw) -> Options:
     o = Options()
     lib().gsofa_options_default(C.byref(o))
@@ -252,7 +285,7 @@ def _stream_handle(stream) -> Optional[int]:
         return None
     if isinstance(stream, int):
         return stream
-    return int(stream.cuda_stream)  # torch.cuda.Stream
+    return int(stream.handle)  # gsofatpds_amd.Stream
 
 
 class Plan:

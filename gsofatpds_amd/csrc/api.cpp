@@ -25,6 +25,7 @@ Options to_options(const gsofa_options* o) {
   r.heavy_wgs_per_cu = o->heavy_wgs_per_cu;
   r.light_wgs_per_cu = o->light_wgs_per_cu;
   r.qcap = o->queue_capacity;
+  r.chain_block = o->chain_block > 0 ? o->chain_block : 64;
   return r;
 }
 
@@ -115,6 +116,30 @@ int gsofa_device_query(int device, int32_t* cus, int64_t* lds_per_block, int64_t
     std::strncpy(arch, p.gcnArchName, (size_t)arch_len - 1);
     arch[arch_len - 1] = '\0';
   }
+  return GSOFA_OK;
+}
+
+int gsofa_device_synchronize(int device) {
+  if (hipSetDevice(device) != hipSuccess) return fail(GSOFA_ERR_NO_DEVICE, "hipSetDevice");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+  return GSOFA_OK;
+}
+
+int gsofa_stream_create(int device, void** out) {
+  if (!out) return fail(GSOFA_ERR_INVALID_ARG, "out is null");
+  if (hipSetDevice(device) != hipSuccess) return fail(GSOFA_ERR_NO_DEVICE, "hipSetDevice");
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  *out = s;
+  return GSOFA_OK;
+}
+
+int gsofa_stream_destroy(void* s) {
+  if (!s) return GSOFA_OK;
+  hipError_t e = hipStreamDestroy(static_cast<hipStream_t>(s));
+  if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipStreamDestroy: ") + hipGetErrorString(e));
   return GSOFA_OK;
 }
 

@@ -20,21 +20,26 @@ int prepare(const gsofa_csr* a, Prepared& out) {
   out.dup_u.clear();
   std::vector<int32_t> row;
   bool any_dup = false;
-  std::vector<int32_t> dup((size_t)n, 0);
+  std::vector<int32_t> dup((size_t)n, 0), dup_all((size_t)n, 0);
   for (int32_t v = 0; v < n; ++v) {
     row.assign(a->adj + a->beg[v], a->adj + a->end[v]);
     std::sort(row.begin(), row.end());
     int32_t upper_all = 0;
     for (int32_t c : row) upper_all += c > v;
+    const int32_t len_in = (int32_t)row.size();
     row.erase(std::unique(row.begin(), row.end()), row.end());
+    dup_all[v] = len_in - (int32_t)row.size();
     int32_t upper_distinct = 0;
     for (int32_t c : row) upper_distinct += c > v;
     dup[v] = upper_all - upper_distinct;
-    any_dup |= dup[v] != 0;
+    any_dup |= dup_all[v] != 0;
     out.adj.insert(out.adj.end(), row.begin(), row.end());
     out.rowptr[v + 1] = (int32_t)out.adj.size();
   }
-  if (any_dup) out.dup_u.swap(dup);
+  if (any_dup) {
+    out.dup_u.swap(dup);
+    out.dup_all.swap(dup_all);
+  }
   return GSOFA_OK;
 }
 
@@ -90,6 +95,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (nmats_ <= 0) return fail(GSOFA_ERR_INVALID_ARG, "empty matrix set");
   if (nmats_ > (1 << 20)) return fail(GSOFA_ERR_CAPACITY, "too many matrices in one set");
   if (opt.chunk <= 0) return fail(GSOFA_ERR_INVALID_ARG, "chunk size must be positive");
+  if (opt.chain_block <= 0) return fail(GSOFA_ERR_INVALID_ARG, "chain block must be positive");
   DeviceInfo info;
   GSOFA_TRY(device_info(opt.device, info));
   GSOFA_HIP(hipSetDevice(opt.device), "hipSetDevice");
@@ -111,6 +117,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
   }
+  blkq_base_h_.assign(nmats_ + 1, 0);
+  for (int b = 0; b < nmats_; ++b)
+    blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt.chain_block - 1) / opt.chain_block;
+  nblk_total_ = blkq_base_h_[nmats_];
   nrows_ = row_base_h_[nmats_];
   nnz_ = nnz_base_h_[nmats_];
   nitems_ = item_base_h_[nmats_];
@@ -168,24 +178,39 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(ws_, 1, "work state"));
   GSOFA_TRY(alloc(heavy_items_, nitems_, "escalation list"));
   GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * 3 * (size_t)n_max_, "heavy spill regions"));
-  if (any_dup) GSOFA_TRY(alloc(dup_u_, nrows_, "duplicate surplus"));
+  esc_words_ = (2 * nrows_ + 3) / 4;
+  GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
+  GSOFA_TRY(alloc(blk_flag_, (size_t)(2 * nblk_total_), "block flags"));
+  if (any_dup) {
+    GSOFA_TRY(alloc(dup_u_, nrows_, "duplicate surplus (upper)"));
+    GSOFA_TRY(alloc(dup_all_, nrows_, "duplicate surplus (row)"));
+  }
 
   // ---- uploads ----
   std::vector<int32_t> rp((size_t)nrows_ + 1);
   std::vector<int32_t> adj_all((size_t)std::max<int64_t>(nnz_, 1));
-  std::vector<int32_t> dup_all;
-  if (any_dup) dup_all.assign((size_t)nrows_, 0);
+  std::vector<int32_t> dup_all, dupr_all;
+  if (any_dup) {
+    dup_all.assign((size_t)nrows_, 0);
+    dupr_all.assign((size_t)nrows_, 0);
+  }
   for (int b = 0; b < nmats_; ++b) {
     const Prepared& P = *mats[b];
     const int64_t rb = row_base_h_[b], nb = nnz_base_h_[b];
     for (int32_t v = 0; v < P.n; ++v) rp[rb + v] = (int32_t)(nb + P.rowptr[v]);
     std::copy(P.adj.begin(), P.adj.end(), adj_all.begin() + nb);
-    if (any_dup && !P.dup_u.empty()) std::copy(P.dup_u.begin(), P.dup_u.end(), dup_all.begin() + rb);
+    if (any_dup && !P.dup_u.empty()) {
+      std::copy(P.dup_u.begin(), P.dup_u.end(), dup_all.begin() + rb);
+      std::copy(P.dup_all.begin(), P.dup_all.end(), dupr_all.begin() + rb);
+    }
   }
   rp[nrows_] = (int32_t)nnz_;
   GSOFA_HIP(hipMemcpy(rowptr_a_, rp.data(), rp.size() * 4, hipMemcpyHostToDevice), "upload rowptr");
   if (nnz_ > 0) GSOFA_HIP(hipMemcpy(adj_a_, adj_all.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice), "upload adj");
-  if (any_dup) GSOFA_HIP(hipMemcpy(dup_u_, dup_all.data(), dup_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
+  if (any_dup) {
+    GSOFA_HIP(hipMemcpy(dup_u_, dup_all.data(), dup_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
+    GSOFA_HIP(hipMemcpy(dup_all_, dupr_all.data(), dupr_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
+  }
   GSOFA_HIP(hipMemcpy(item_base_d_, item_base_h_.data(), item_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
   GSOFA_HIP(hipMemcpy(row_base_d_, row_base_h_.data(), row_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
   GSOFA_HIP(hipMemcpy(blk_base_d_, blk_base_h_.data(), blk_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
@@ -206,10 +231,16 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.cnt[1] = cnt_l_ + rb;
     M.win = win_ + rb * 4;
     M.dup_u = dup_u_ ? dup_u_ + rb : nullptr;
+    M.dup_all = dup_all_ ? dup_all_ + rb : nullptr;
     M.list_ptr[0] = M.list_ptr[1] = nullptr;
     M.pool[0] = M.pool[1] = nullptr;
     M.sup_flag = sup_flag_ + rb;
     M.acc = acc_ + (size_t)b * kAccSlots;
+    M.esc[0] = esc_ + rb;
+    M.esc[1] = esc_ + nrows_ + rb;
+    M.blk_flag[0] = blk_flag_ + blkq_base_h_[b];
+    M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
+    M.chain_block = opt.chain_block;
   }
   GSOFA_HIP(hipMemcpy(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, hipMemcpyHostToDevice), "upload table");
   return GSOFA_OK;
@@ -222,6 +253,8 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   if (ev) GSOFA_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
   GSOFA_HIP(launch_fill_u64(acc_, (int64_t)nmats_ * kAccSlots, 0ull, s), "clear accumulators");
   GSOFA_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(ws_), 4, 0ull, s), "clear work state");
+  GSOFA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(esc_), esc_words_, 0, s), "clear escalation flags");
+  GSOFA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(blk_flag_), 2 * nblk_total_, 0, s), "clear block flags");
   GSOFA_HIP(launch_ell_build(nrows_, rowptr_a_, adj_a_, ell_a_, s), "ell A");
   GSOFA_HIP(launch_transpose(nrows_, nmats_, row_base_d_, rowptr_a_, adj_a_, col_cnt_, rowptr_t_, cursor_, adj_t_,
                              block_sums_, total_, s),

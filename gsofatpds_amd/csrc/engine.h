@@ -20,6 +20,7 @@ struct Prepared {
   std::vector<int32_t> rowptr;  // n + 1
   std::vector<int32_t> adj;
   std::vector<int32_t> dup_u;   // empty when the input had no duplicate upper entries
+  std::vector<int32_t> dup_all; // input row length minus de-duplicated length; empty if none
 };
 
 int prepare(const gsofa_csr* a, Prepared& out);
@@ -33,6 +34,7 @@ struct Options {
   int32_t heavy_wgs_per_cu = 0; // 0 = tuned
   int32_t light_wgs_per_cu = 0; // 0 = tuned
   int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
+  int32_t chain_block = 64;     // rows per workgroup-path item; 1 disables chaining
 };
 
 struct MatResult {
@@ -95,7 +97,7 @@ class DeviceSet {
   int32_t *rowptr_a_ = nullptr, *adj_a_ = nullptr, *ell_a_ = nullptr;
   int32_t *col_cnt_ = nullptr, *rowptr_t_ = nullptr, *cursor_ = nullptr, *adj_t_ = nullptr, *ell_t_ = nullptr;
   int32_t *block_sums_ = nullptr, *total_ = nullptr;
-  int32_t *cnt_u_ = nullptr, *cnt_l_ = nullptr, *dup_u_ = nullptr;
+  int32_t *cnt_u_ = nullptr, *cnt_l_ = nullptr, *dup_u_ = nullptr, *dup_all_ = nullptr;
   uint32_t* win_ = nullptr;
   uint8_t* sup_flag_ = nullptr;
   unsigned long long* acc_ = nullptr;
@@ -104,6 +106,10 @@ class DeviceSet {
   WorkState* ws_ = nullptr;
   uint64_t* heavy_items_ = nullptr;
   int32_t* spill_ = nullptr;
+  uint8_t* esc_ = nullptr;        // 2 * nrows escalation flags (padded to a multiple of 4)
+  uint32_t* blk_flag_ = nullptr;  // 2 * nblk_total
+  int64_t nblk_total_ = 0, esc_words_ = 0;
+  std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries
   int32_t *pool_u_ = nullptr, *pool_l_ = nullptr;
   std::vector<int64_t> ptr_base_h_;                        // matrix b's offsets start at row_base + b

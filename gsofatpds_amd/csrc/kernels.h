@@ -39,10 +39,14 @@ struct MatDev {
   int32_t* cnt[2];                // n: |U_i| (dir 0) / |L column j| (dir 1)
   uint32_t* win;                  // n * 4: bits t of column j: row j+1+t in L, same chunk
   const int32_t* dup_u;           // n: duplicate surplus of A's upper entries (may be null)
+  const int32_t* dup_all;         // n: duplicate surplus of A's rows, for W (may be null)
   const int64_t* list_ptr[2];     // n + 1: row/column offsets into pool (null: no structure)
   int32_t* pool[2];               // strict U rows (dir 0) / strict L columns (dir 1)
   uint8_t* sup_flag;              // n: 1 where the supernode count increments
   unsigned long long* acc;        // kAccSlots
+  uint8_t* esc[2];                // n: 1 where the wave path escalated the source
+  uint32_t* blk_flag[2];          // ceil(n / chain_block): block already queued for the workgroup path
+  int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
 };
 
 // Scheduler state of one launch set (plain matrix or batch).
@@ -53,9 +57,9 @@ struct WorkState {
   unsigned long long pad;
 };
 
-// Item encoding for the escalation list: matrix index, direction, source row.
-__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t src) {
-  return ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)src;
+// Item encoding for the escalation list: matrix index, direction, block of chain_block rows.
+__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t block) {
+  return ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)block;
 }
 
 }  // namespace gsofa

@@ -393,6 +393,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
         if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
         visit(m);
       }
+      if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
       w_acc += deg;
       if (esc) break;
       if (seed) {
@@ -405,8 +406,12 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
 
     if (esc) {
       if (lane == 0) {
-        const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
-        heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)src);
+        gstore(M.esc[dir] + src, (uint8_t)1);
+        const int32_t blk = src / M.chain_block;
+        if (atomicOr(&M.blk_flag[dir][blk], 1u) == 0u) {
+          const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
+          heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)blk);
+        }
       }
       for (int i = lane; i < kHash; i += 64) H[i] = -1;
       continue;
@@ -486,6 +491,12 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
   t = Totals{};
 }
 
+// One workgroup per item = a block of chain_block consecutive rows of one direction; it
+// traverses the block's escalated rows in increasing order. Chaining: when row s holds s-1 and
+// s-1 was the previous row, s-1 is in R_s, so R_{s-1} and everything adjacent to it is in the
+// reached set of s (X_{s-1} is a subset of X_s). The bitmap, the U list (entries > s kept) and
+// the work counters then carry over and only vertices new to s are expanded. Otherwise the
+// bitmap is cleared and s is traversed from scratch. Results are identical either way.
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ mats,
                                                     WorkState* __restrict__ ws,
@@ -493,28 +504,29 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
                                                     int32_t bw_max, int32_t qcap, int64_t n_max,
                                                     int32_t* __restrict__ spill_all) {
   extern __shared__ uint32_t dyn[];
+  constexpr int kWaves = kThreads / 64;
   __shared__ unsigned long long s_idx;
   __shared__ int32_t s_cnt[3];
-  __shared__ uint32_t s_u, s_useed, s_ust;
+  __shared__ uint32_t s_ust, s_useed;
   __shared__ uint32_t s_win[4];
   __shared__ unsigned long long s_w;
   __shared__ int64_t s_off;
+  __shared__ int32_t s_chain, s_deg0;
+  __shared__ uint32_t s_scan[kWaves];
 
-  constexpr int kWaves = kThreads / 64;
   const int tid = threadIdx.x;
   const int lane = lane_id();
+  const int wave = tid >> 6;
   uint32_t* bm = dyn;
   int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
   int32_t* const ql1 = ql0 + qcap;
   int32_t* const region = spill_all + (int64_t)blockIdx.x * 3 * n_max;
   int32_t* const qg0 = region;
   int32_t* const qg1 = region + n_max;
-  int32_t* const ustage = region + 2 * n_max;
-  (void)kWaves;
+  int32_t* const ustage = region + 2 * n_max;  // U list of the current source, carried on chaining
 
-  // thread 0 keeps per-matrix totals for the matrix it last worked on
   int last_mat = -1;
-  Totals tot;
+  Totals tot;  // thread 0 only
 
   const unsigned long long total = ws->heavy_count;
   while (true) {
@@ -525,76 +537,123 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
     const uint64_t item = heavy_items[idx];
     const int mb = (int)(item >> 33);
     const int dir = (int)((item >> 32) & 1);
-    const int32_t src = (int32_t)(item & 0xffffffffu);
+    const int32_t blk = (int32_t)(item & 0xffffffffu);
     const MatDev& M = mats[mb];
     const int32_t n = M.n;
     const int32_t bw = (n + 31) >> 5;
-    const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
+    const int32_t r0 = blk * M.chain_block;
+    const int32_t r1 = min(n, r0 + M.chain_block);
     const bool want_list = M.list_ptr[dir] != nullptr;
 
-    for (int32_t w = tid; w < bw; w += kThreads) bm[w] = 0u;
-    if (tid == 0) {
-      s_cnt[0] = 1;
-      s_cnt[1] = 0;
-      s_cnt[2] = 0;
-      s_u = s_useed = s_ust = 0;
-      s_win[0] = s_win[1] = s_win[2] = s_win[3] = 0;
-      s_w = 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      bm[src >> 5] |= 1u << (src & 31);
-      ql0[0] = src;
-    }
-    __syncthreads();
+    int32_t prev = -1;                    // previous row traversed in this block
+    uint64_t wr_prev = 0, deg_prev = 0;   // W_R and input degree of prev
+    uint64_t r_prev = 0;                  // |R| of prev
+    for (int32_t src = r0; src < r1; ++src) {
+      if (gload(M.esc[dir] + src) == 0) continue;  // the wave path finished this row
+      const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
 
-    uint64_t w_acc = 0;
-    uint32_t u_cnt = 0, u_seed = 0;
-    uint32_t win[4] = {0, 0, 0, 0};
-    int64_t r_total = 0;
-    for (int level = 0;; ++level) {
-      const int32_t ncur = s_cnt[level % 3];
-      if (ncur == 0) break;
-      if (level > 0) r_total += ncur;
-      if (tid == 0) s_cnt[(level + 2) % 3] = 0;
-      const bool odd = (level & 1) != 0;
-      const int32_t* const cur_l = odd ? ql1 : ql0;
-      const int32_t* const cur_g = odd ? qg1 : qg0;
-      int32_t* const nxt_l = odd ? ql0 : ql1;
-      int32_t* const nxt_g = odd ? qg0 : qg1;
-      int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
-      const bool seed = level == 0;
-      for (int32_t base = 0; base < ncur; base += kThreads) {
-        const int32_t e = base + tid;
-        const int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
-        RowCursor rc;
-        load_row(M, dir, v, rc);
-        uint32_t deg = 0;
-        auto visit = [&](int32_t m) {
-          const bool active = m >= 0;
-          deg += active;
-          bool isnew = false;
-          if (active) {
-            const uint32_t bit = 1u << (m & 31);
-            uint32_t* wp = &bm[m >> 5];
-            if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+      // ---- chain test: does row src hold src - 1? (wave 0) ----
+      if (tid == 0) s_chain = 0;
+      __syncthreads();
+      if (prev == src - 1 && wave == 0) {
+        const int32_t* row = M.ell[dir] + (int64_t)src * kEll;
+        int32_t e = lane < kEll ? gload(row + lane) : kPad;
+        bool hit = e == src - 1;
+        if (lane == kEll - 1 && e <= -2) {  // long row: scan its CSR tail
+          hit = false;
+          for (int32_t p = -(e + 2), pe = gload(M.rowptr[dir] + src + 1); p < pe; ++p)
+            hit |= gload(M.adj[dir] + p) == src - 1;
+        }
+        if (__ballot(hit) != 0 && lane == 0) s_chain = 1;
+      }
+      __syncthreads();
+      const bool chained = s_chain != 0;
+
+      uint64_t carried_wr = 0, carried_r = 0;
+      if (!chained) {
+        for (int32_t w = tid; w < bw; w += kThreads) bm[w] = 0u;
+        if (tid == 0) s_ust = 0;
+      } else {
+        carried_wr = wr_prev + deg_prev;
+        carried_r = r_prev + 1;
+        // keep the carried U entries above src, in place (reads of a round finish before its writes)
+        const uint32_t len = s_ust;
+        uint32_t kept = 0;
+        for (uint32_t base = 0; base < len; base += kThreads) {
+          const uint32_t i = base + tid;
+          const int32_t m = i < len ? gload(ustage + i) : -1;
+          const bool keep = m > src;
+          const uint64_t bal = __ballot(keep);
+          if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+          __syncthreads();
+          uint32_t before = 0, all = 0;
+          for (int w = 0; w < kWaves; ++w) {
+            if (w < wave) before += s_scan[w];
+            all += s_scan[w];
           }
-          const bool is_r = isnew && m < src;
-          const bool is_u = isnew && m > src;
-          const uint64_t mr = __ballot(is_r);
-          if (mr) {
-            const int leader = __ffsll((unsigned long long)mr) - 1;
-            int32_t basepos = 0;
-            if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
-            basepos = __shfl(basepos, leader, 64);
-            if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
-          }
-          if (is_u) {
-            ++u_cnt;
-            if (seed) ++u_seed;
-            if (dir == 1) window_bit(win, src, m, chunk_end);
-          }
-          if (want_list) {
+          if (keep) gstore(ustage + kept + before + rank_below(bal), m);
+          kept += all;
+          __syncthreads();
+        }
+        if (tid == 0) s_ust = kept;
+      }
+      if (tid == 0) {
+        s_cnt[0] = 1;
+        s_cnt[1] = 0;
+        s_cnt[2] = 0;
+        s_useed = 0;
+        s_win[0] = s_win[1] = s_win[2] = s_win[3] = 0;
+        s_w = 0;
+        s_deg0 = 0;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        atomicOr(&bm[src >> 5], 1u << (src & 31));
+        ql0[0] = src;
+      }
+      __syncthreads();
+
+      uint64_t w_acc = 0;
+      uint32_t u_seed = 0;
+      int64_t r_new = 0;
+      for (int level = 0;; ++level) {
+        const int32_t ncur = s_cnt[level % 3];
+        if (ncur == 0) break;
+        if (level > 0) r_new += ncur;
+        if (tid == 0) s_cnt[(level + 2) % 3] = 0;
+        const bool odd = (level & 1) != 0;
+        const int32_t* const cur_l = odd ? ql1 : ql0;
+        const int32_t* const cur_g = odd ? qg1 : qg0;
+        int32_t* const nxt_l = odd ? ql0 : ql1;
+        int32_t* const nxt_g = odd ? qg0 : qg1;
+        int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
+        const bool seed = level == 0;
+        for (int32_t base = 0; base < ncur; base += kThreads) {
+          const int32_t e = base + tid;
+          const int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+          RowCursor rc;
+          load_row(M, dir, v, rc);
+          uint32_t deg = 0;
+          auto visit = [&](int32_t m) {
+            const bool active = m >= 0;
+            deg += active;
+            if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
+            bool isnew = false;
+            if (active) {
+              const uint32_t bit = 1u << (m & 31);
+              uint32_t* wp = &bm[m >> 5];
+              if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+            }
+            const bool is_r = isnew && m < src;
+            const bool is_u = isnew && m > src;
+            const uint64_t mr = __ballot(is_r);
+            if (mr) {
+              const int leader = __ffsll((unsigned long long)mr) - 1;
+              int32_t basepos = 0;
+              if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
+              basepos = __shfl(basepos, leader, 64);
+              if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
+            }
             const uint64_t mu = __ballot(is_u);
             if (mu) {
               const int leader = __ffsll((unsigned long long)mu) - 1;
@@ -603,75 +662,87 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
               basepos = __shfl(basepos, leader, 64);
               if (is_u) gstore(ustage + basepos + rank_below(mu), m);
             }
-          }
-        };
+          };
 #pragma unroll
-        for (int j = 0; j < kEll; ++j) {
-          if (__ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+          for (int j = 0; j < kEll; ++j) {
+            if (__ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+          }
+          while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+            int32_t m = kPad;
+            if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
+            visit(m);
+          }
+          if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
+          w_acc += deg;
+          if (seed && tid == 0) s_deg0 = (int32_t)deg;
         }
-        while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
-          int32_t m = kPad;
-          if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
-          visit(m);
+        __syncthreads();
+      }
+
+      // ---- this source's totals ----
+      {
+        const uint64_t us = wave_sum_u64(u_seed), wa = wave_sum_u64(w_acc);
+        if (lane == 0) {
+          if (us) atomicAdd(&s_useed, (uint32_t)us);
+          if (wa) atomicAdd(&s_w, (unsigned long long)wa);
         }
-        w_acc += deg;
+      }
+      const uint32_t U = s_ust;  // stable: the level loop ended with a barrier
+      if (dir == 1) {
+        uint32_t win[4] = {0, 0, 0, 0};
+        for (uint32_t i = tid; i < U; i += kThreads) window_bit(win, src, gload(ustage + i), chunk_end);
+        const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
+        const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
+        if (lane == 0) {
+          if (w0) atomicOr(&s_win[0], w0);
+          if (w1) atomicOr(&s_win[1], w1);
+          if (w2) atomicOr(&s_win[2], w2);
+          if (w3) atomicOr(&s_win[3], w3);
+        }
       }
       __syncthreads();
-    }
-
-    // reduce the per-thread counters of this source
-    {
-      const uint64_t uc = wave_sum_u64(u_cnt), us = wave_sum_u64(u_seed), wa = wave_sum_u64(w_acc);
-      const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
-      const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
-      if (lane == 0) {
-        if (uc) atomicAdd(&s_u, (uint32_t)uc);
-        if (us) atomicAdd(&s_useed, (uint32_t)us);
-        if (wa) atomicAdd(&s_w, (unsigned long long)wa);
-        if (w0) atomicOr(&s_win[0], w0);
-        if (w1) atomicOr(&s_win[1], w1);
-        if (w2) atomicOr(&s_win[2], w2);
-        if (w3) atomicOr(&s_win[3], w3);
-      }
-    }
-    __syncthreads();
-    const uint32_t U = s_u;
-    if (tid == 0) {
-      if (mb != last_mat) {
-        flush_totals(mats, last_mat, tot);
-        last_mat = mb;
-      }
-      M.cnt[dir][src] = (int32_t)U;
-      tot.fill += U - s_useed;
-      tot.heavy += 1;
-      if (dir == 0) {
-        tot.w += s_w;
-        tot.r += (uint64_t)r_total;
-        tot.nnz_u += U;
-      } else {
-        tot.nnz_l += U;
-        reinterpret_cast<uint4*>(M.win)[src] = make_uint4(s_win[0], s_win[1], s_win[2], s_win[3]);
+      const uint64_t w_src = s_w;
+      const uint64_t deg0 = (uint64_t)s_deg0;
+      if (tid == 0) {
+        if (mb != last_mat) {
+          flush_totals(mats, last_mat, tot);
+          last_mat = mb;
+        }
+        M.cnt[dir][src] = (int32_t)U;
+        tot.fill += U - s_useed;
+        tot.heavy += 1;
+        if (dir == 0) {
+          tot.w += w_src + carried_wr;
+          tot.r += (uint64_t)r_new + carried_r;
+          tot.nnz_u += U;
+        } else {
+          tot.nnz_l += U;
+          reinterpret_cast<uint4*>(M.win)[src] = make_uint4(s_win[0], s_win[1], s_win[2], s_win[3]);
+        }
+        if (want_list) {
+          int64_t off = gload(M.list_ptr[dir] + src);
+          if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U) {
+            atomicOr(&M.acc[kAccOverflow], 1ull);
+            off = -1;
+          }
+          s_off = off;
+        }
       }
       if (want_list) {
-        int64_t off = gload(M.list_ptr[dir] + src);
-        if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U) {
-          atomicOr(&M.acc[kAccOverflow], 1ull);
-          off = -1;
-        }
-        s_off = off;
+        __syncthreads();
+        const int64_t off = s_off;
+        if (off >= 0)
+          for (uint32_t i = tid; i < U; i += kThreads) gstore(M.pool[dir] + off + i, gload(ustage + i));
       }
-    }
-    if (want_list) {
+      wr_prev = carried_wr + (w_src - deg0);
+      deg_prev = deg0;
+      r_prev = carried_r + (uint64_t)r_new;
+      prev = src;
       __syncthreads();
-      const int64_t off = s_off;
-      if (off >= 0)
-        for (uint32_t i = tid; i < U; i += kThreads) gstore(M.pool[dir] + off + i, gload(ustage + i));
     }
-    __syncthreads();
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
 }
-
 
 // ------------------------------------------------------------------------------------------
 // k_supernodes: the reference's rule (warpcentric_local_detection_test,

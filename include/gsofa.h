@@ -96,6 +96,13 @@ int gsofa_symbfact_min_id(int argc, char** argv, int rank, uint64_t* fill_count,
 int gsofa_device_query(int device, int32_t* cus, int64_t* lds_per_block, int64_t* lds_per_cu, char* arch,
                        int32_t arch_len);
 
+/* Stream and synchronisation helpers for callers without a HIP runtime of their own (a process
+ * must not load a second HIP runtime, e.g. a framework's bundled one, next to this library's).
+ * No counterpart in the reference. */
+int gsofa_device_synchronize(int device);
+int gsofa_stream_create(int device, void** hip_stream);
+int gsofa_stream_destroy(void* hip_stream);
+
 /* ---- structured API --------------------------------------------------------------------- */
 typedef struct gsofa_options {
   int32_t chunk_size;       /* supernode chunk S (argv[5]); default 128                      */
@@ -106,6 +113,7 @@ typedef struct gsofa_options {
   int32_t heavy_wgs_per_cu; /* tuning: resident bitmap workgroups per CU, 0 = automatic         */
   int32_t light_wgs_per_cu; /* tuning: resident hash-path workgroups per CU, 0 = automatic      */
   int32_t queue_capacity;   /* tuning: LDS frontier entries per level buffer, 0 = automatic     */
+  int32_t chain_block;      /* tuning: rows per workgroup-path item (1 = no chaining), 0 = 64   */
 } gsofa_options;
 
 typedef struct gsofa_result {

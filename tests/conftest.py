@@ -16,11 +16,10 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    # through the engine's own runtime: loading a second HIP runtime (a framework's) in the same
+    # process would take the device away from the engine
+    import gsofatpds_amd
+    return gsofatpds_amd.gpu_present()
 
 
 @pytest.fixture(scope="session")

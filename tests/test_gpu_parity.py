@@ -127,11 +127,20 @@ def test_batch_members(member):
     check(p, structure=p.n < 3000, label=f"batch member {member}")
 
 
+@pytest.mark.parametrize("chain_block", [1, 3, 64, 4096])
+def test_chaining_blocks_grid3d_small(chain_block):
+    """ND 3-D stencil (separators chain along grid lines) with several block sizes."""
+    p = g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=14, ny=14, nz=14, order=g.ORDER_ND,
+                   drop_side=g.DROP_UPPER, drop_ppm=150000, seed=77)
+    check(p, S=128, label=f"grid3d 14^3 chain {chain_block}", chain_block=chain_block)
+
+
 def test_tuning_configurations_are_identical():
     p = load_ds()
     base = None
     for opt in (dict(), dict(heavy_threads=256), dict(heavy_threads=1024), dict(queue_capacity=256),
-                dict(light_wgs_per_cu=1, heavy_wgs_per_cu=1)):
+                dict(light_wgs_per_cu=1, heavy_wgs_per_cu=1), dict(chain_block=1), dict(chain_block=7),
+                dict(chain_block=1024, queue_capacity=256)):
         say(f"[tuning] {opt}")
         with g.Plan(p, **opt) as plan:
             plan.run()
@@ -144,9 +153,8 @@ def test_tuning_configurations_are_identical():
 
 
 def test_repeated_runs_and_stream():
-    import torch
     p = g.generate_workload(g.WL_GRID2D_4096)
-    s = torch.cuda.Stream()
+    s = g.Stream()
     with g.Plan(p, structure=1) as plan:
         plan.run(s).sync()
         first = plan.result().as_dict()
