@@ -113,6 +113,18 @@ _SIGNATURES = {
     "gsofa_batch_nzu_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "gsofa_batch_supernodes_get": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint8)]),
     "gsofa_batch_destroy": (None, [C.c_void_p]),
+    "gsofa_server_options_default": (None, [C.c_void_p]),
+    "gsofa_server_start": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "gsofa_server_submit": (C.c_int, [C.c_void_p, C.POINTER(CSR), C.POINTER(Options), C.POINTER(C.c_int64)]),
+    "gsofa_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result)]),
+    "gsofa_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(Result)]),
+    "gsofa_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
+    "gsofa_server_request_info": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "gsofa_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]),
+    "gsofa_server_stop": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -399,3 +411,71 @@ class Batch:
         out = np.zeros(self.patterns[i].n, np.uint8)
         _check(lib().gsofa_batch_supernodes_get(self._h, i, out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
+
+
+class Server:
+    """Continuous-batching request loop: ``gsofa_server_*``."""
+
+    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 0, device: int = 0):
+        o = ServerOptions()
+        lib().gsofa_server_options_default(C.byref(o))
+        o.slots, o.hbm_budget_bytes, o.device = slots, hbm_budget_bytes, device
+        h = C.c_void_p()
+        _check(lib().gsofa_server_start(C.byref(o), C.byref(h)))
+        self._h = h
+
+    def submit(self, pattern: Pattern, **options) -> int:
+        c = pattern.csr()
+        o = default_options(**options)
+        rid = C.c_int64()
+        _check(lib().gsofa_server_submit(self._h, C.byref(c), C.byref(o), C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int) -> Optional[Result]:
+        r = Result()
+        rc = lib().gsofa_server_poll(self._h, rid, C.byref(r))
+        if rc == ERR_NOT_READY:
+            return None
+        _check(rc)
+        return r
+
+    def wait(self, rid: int, timeout_s: float = -1.0) -> Result:
+        r = Result()
+        _check(lib().gsofa_server_wait(self._h, rid, timeout_s, C.byref(r)))
+        return r
+
+    def cancel(self, rid: int) -> None:
+        _check(lib().gsofa_server_cancel(self._h, rid))
+
+    def info(self, rid: int) -> dict:
+        lat, q = C.c_double(), C.c_double()
+        dz, ds = C.c_uint64(), C.c_uint64()
+        _check(lib().gsofa_server_request_info(self._h, rid, C.byref(lat), C.byref(q), C.byref(dz), C.byref(ds)))
+        return {"latency_ms": lat.value, "queue_ms": q.value, "nzu_digest": dz.value, "supernode_digest": ds.value}
+
+    def stats(self) -> dict:
+        c, pn, iu, b = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        r, pr = C.c_int32(), C.c_int32()
+        _check(lib().gsofa_server_stats(self._h, C.byref(c), C.byref(r), C.byref(pr), C.byref(pn), C.byref(iu),
+                                        C.byref(b)))
+        return {"completed": c.value, "running": r.value, "peak_running": pr.value, "pending": pn.value,
+                "hbm_in_use": iu.value, "hbm_budget": b.value}
+
+    def stop(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gsofa_server_stop(self._h)
+            self._h = None
+
+    __del__ = stop
+
+
+def fnv1a(data: bytes) -> int:
+    """The digest the server reports for nzU and the supernode flags."""
+    h = 1469598103934665603
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgsofa.so")
 CLI = os.path.join(HERE, "gsofa_cli")
 
-SOURCES = ["kernels.hip", "engine.cpp", "api.cpp", "host.cpp", "gen.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "api.cpp", "server.cpp", "host.cpp", "gen.cpp"]
 # -packed-fp32-ops: no v_pk_*_f32 generation on this pool (the engine has no fp32 math anyway).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -59,7 +59,31 @@ int device_info(int device, DeviceInfo& info) {
 }
 
 DeviceSet::~DeviceSet() {
-  for (void* p : owned_) (void)hipFree(p);
+  for (void* p : owned_) {
+    if (alloc_stream_) (void)hipFreeAsync(p, alloc_stream_);
+    else (void)hipFree(p);
+  }
+}
+
+int DeviceSet::download(void* dst, const void* src, size_t bytes, const char* what) {
+  if (bytes == 0) return GSOFA_OK;
+  hipError_t e;
+  if (alloc_stream_) {
+    e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, alloc_stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(alloc_stream_);
+  } else {
+    e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) return hip_fail(e, what);
+  return GSOFA_OK;
+}
+
+int DeviceSet::upload(void* dst, const void* src, size_t bytes, const char* what) {
+  if (bytes == 0) return GSOFA_OK;
+  hipError_t e = alloc_stream_ ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, alloc_stream_)
+                               : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return GSOFA_OK;
 }
 
 int DeviceSet::hip_fail(hipError_t e, const char* what) {
@@ -71,7 +95,7 @@ template <typename T>
 int DeviceSet::alloc(T*& p, size_t count, const char* what) {
   void* q = nullptr;
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&q, bytes);
+  hipError_t e = alloc_stream_ ? hipMallocAsync(&q, bytes, alloc_stream_) : hipMalloc(&q, bytes);
   if (e != hipSuccess) return hip_fail(e, what);
   owned_.push_back(q);
   p = static_cast<T*>(q);
@@ -89,8 +113,9 @@ int DeviceSet::alloc(T*& p, size_t count, const char* what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
-int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt) {
+int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream) {
   opt_ = opt;
+  alloc_stream_ = alloc_stream;
   nmats_ = (int)mats.size();
   if (nmats_ <= 0) return fail(GSOFA_ERR_INVALID_ARG, "empty matrix set");
   if (nmats_ > (1 << 20)) return fail(GSOFA_ERR_CAPACITY, "too many matrices in one set");
@@ -150,10 +175,11 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds), "hipFuncSetAttribute(k_heavy)");
   const int occ = heavy_occupancy(heavy_threads_, lds);
   if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
-  heavy_grid_ = info.cus * occ;
+  const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
+  heavy_grid_ = cus * occ;
   const int occ_l = light_occupancy();
   if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
-  light_grid_ = info.cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
+  light_grid_ = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
 
   // ---- allocations ----
   GSOFA_TRY(alloc(rowptr_a_, nrows_ + 1, "rowptr A"));
@@ -205,15 +231,19 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     }
   }
   rp[nrows_] = (int32_t)nnz_;
-  GSOFA_HIP(hipMemcpy(rowptr_a_, rp.data(), rp.size() * 4, hipMemcpyHostToDevice), "upload rowptr");
-  if (nnz_ > 0) GSOFA_HIP(hipMemcpy(adj_a_, adj_all.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice), "upload adj");
+  GSOFA_TRY(upload(rowptr_a_, rp.data(), rp.size() * 4, "upload rowptr"));
+  GSOFA_TRY(upload(adj_a_, adj_all.data(), (size_t)nnz_ * 4, "upload adj"));
   if (any_dup) {
-    GSOFA_HIP(hipMemcpy(dup_u_, dup_all.data(), dup_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
-    GSOFA_HIP(hipMemcpy(dup_all_, dupr_all.data(), dupr_all.size() * 4, hipMemcpyHostToDevice), "upload dup");
+    GSOFA_TRY(upload(dup_u_, dup_all.data(), dup_all.size() * 4, "upload dup"));
+    GSOFA_TRY(upload(dup_all_, dupr_all.data(), dupr_all.size() * 4, "upload dup"));
   }
-  GSOFA_HIP(hipMemcpy(item_base_d_, item_base_h_.data(), item_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
-  GSOFA_HIP(hipMemcpy(row_base_d_, row_base_h_.data(), row_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
-  GSOFA_HIP(hipMemcpy(blk_base_d_, blk_base_h_.data(), blk_base_h_.size() * 8, hipMemcpyHostToDevice), "upload");
+  GSOFA_TRY(upload(item_base_d_, item_base_h_.data(), item_base_h_.size() * 8, "upload"));
+  GSOFA_TRY(upload(row_base_d_, row_base_h_.data(), row_base_h_.size() * 8, "upload"));
+  GSOFA_TRY(upload(blk_base_d_, blk_base_h_.data(), blk_base_h_.size() * 8, "upload"));
+  if (alloc_stream_) {
+    // pageable sources: make sure the copies consumed them before the vectors go away
+    GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(uploads)");
+  }
 
   mats_h_.assign(nmats_, MatDev{});
   for (int b = 0; b < nmats_; ++b) {
@@ -242,7 +272,39 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
     M.chain_block = opt.chain_block;
   }
-  GSOFA_HIP(hipMemcpy(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, hipMemcpyHostToDevice), "upload table");
+  GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
+  if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");
+  return GSOFA_OK;
+}
+
+int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Options& opt, int64_t* out_bytes) {
+  // mirrors the allocations of create() (spill sized for the resident heavy grid)
+  DeviceInfo info;
+  int rc = device_info(opt.device, info);
+  if (rc != GSOFA_OK) return rc;
+  int64_t nrows = 0, nnz = 0, nblk = 0;
+  int32_t nmax = 0;
+  bool dup = false;
+  for (const Prepared* P : mats) {
+    nrows += P->n;
+    nnz += (int64_t)P->adj.size();
+    nmax = std::max(nmax, P->n);
+    nblk += ((int64_t)P->n + opt.chain_block - 1) / std::max(opt.chain_block, 1);
+    dup |= !P->dup_u.empty();
+  }
+  const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
+  const int64_t heavy_grid = (int64_t)cus * 2;  // upper bound on resident workgroups per CU
+  int64_t b = 0;
+  b += 4 * (nrows + 1) * 3;           // rowptr A, rowptr A^T, cursor
+  b += 4 * nnz * 2;                   // adj, adj^T
+  b += 4 * nrows * kEll * 2;          // ELL A, A^T
+  b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
+  b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
+  b += 4 * heavy_grid * 3 * (int64_t)nmax;                // spill regions
+  if (dup) b += 8 * nrows;
+  if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
+  b += 1 << 20;                       // tables, accumulators, scan scratch
+  *out_bytes = b;
   return GSOFA_OK;
 }
 
@@ -273,7 +335,7 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
 
 int DeviceSet::read_results(std::vector<MatResult>& out) {
   std::vector<unsigned long long> acc((size_t)nmats_ * kAccSlots);
-  GSOFA_HIP(hipMemcpy(acc.data(), acc_, acc.size() * 8, hipMemcpyDeviceToHost), "read accumulators");
+  GSOFA_TRY(download(acc.data(), acc_, acc.size() * 8, "read accumulators"));
   out.assign(nmats_, MatResult{});
   for (int b = 0; b < nmats_; ++b) {
     const unsigned long long* a = acc.data() + (size_t)b * kAccSlots;
@@ -299,16 +361,16 @@ int DeviceSet::read_counts(int mat, int32_t* cnt_u, int32_t* cnt_l, uint8_t* sup
   const size_t n = (size_t)ns_[mat];
   if (cnt_u || nzu) {
     std::vector<int32_t> tmp(n);
-    GSOFA_HIP(hipMemcpy(tmp.data(), cnt_u_ + rb, n * 4, hipMemcpyDeviceToHost), "read U counts");
+    GSOFA_TRY(download(tmp.data(), cnt_u_ + rb, n * 4, "read U counts"));
     if (cnt_u) std::copy(tmp.begin(), tmp.end(), cnt_u);
     if (nzu) {
       std::vector<int32_t> dup(n, 0);
-      if (dup_u_) GSOFA_HIP(hipMemcpy(dup.data(), dup_u_ + rb, n * 4, hipMemcpyDeviceToHost), "read dup");
+      if (dup_u_) GSOFA_TRY(download(dup.data(), dup_u_ + rb, n * 4, "read dup"));
       for (size_t i = 0; i < n; ++i) nzu[i] = tmp[i] + dup[i];
     }
   }
-  if (cnt_l) GSOFA_HIP(hipMemcpy(cnt_l, cnt_l_ + rb, n * 4, hipMemcpyDeviceToHost), "read L counts");
-  if (sup_flag) GSOFA_HIP(hipMemcpy(sup_flag, sup_flag_ + rb, n, hipMemcpyDeviceToHost), "read supernode flags");
+  if (cnt_l) GSOFA_TRY(download(cnt_l, cnt_l_ + rb, n * 4, "read L counts"));
+  if (sup_flag) GSOFA_TRY(download(sup_flag, sup_flag_ + rb, n, "read supernode flags"));
   return GSOFA_OK;
 }
 
@@ -323,8 +385,8 @@ int DeviceSet::size_pools_from_counts() {
   u_ptr_h_.assign(nmats_, {});
   l_ptr_h_.assign(nmats_, {});
   std::vector<int32_t> cu((size_t)nrows_), cl((size_t)nrows_);
-  GSOFA_HIP(hipMemcpy(cu.data(), cnt_u_, cu.size() * 4, hipMemcpyDeviceToHost), "read U counts");
-  GSOFA_HIP(hipMemcpy(cl.data(), cnt_l_, cl.size() * 4, hipMemcpyDeviceToHost), "read L counts");
+  GSOFA_TRY(download(cu.data(), cnt_u_, cu.size() * 4, "read U counts"));
+  GSOFA_TRY(download(cl.data(), cnt_l_, cl.size() * 4, "read L counts"));
   for (int b = 0; b < nmats_; ++b) {
     const int64_t rb = row_base_h_[b], pb = ptr_base_h_[b];
     int64_t su = 0, sl = 0;
@@ -347,8 +409,8 @@ int DeviceSet::size_pools_from_counts() {
     GSOFA_TRY(alloc(pool_u_, (size_t)pool_base_u_[nmats_], "U structure"));
     GSOFA_TRY(alloc(pool_l_, (size_t)pool_base_l_[nmats_], "L structure"));
   }
-  GSOFA_HIP(hipMemcpy(list_ptr_u_, up.data(), up.size() * 8, hipMemcpyHostToDevice), "upload U offsets");
-  GSOFA_HIP(hipMemcpy(list_ptr_l_, lp.data(), lp.size() * 8, hipMemcpyHostToDevice), "upload L offsets");
+  GSOFA_TRY(upload(list_ptr_u_, up.data(), up.size() * 8, "upload U offsets"));
+  GSOFA_TRY(upload(list_ptr_l_, lp.data(), lp.size() * 8, "upload L offsets"));
   for (int b = 0; b < nmats_; ++b) {
     MatDev& M = mats_h_[b];
     M.list_ptr[0] = list_ptr_u_ + ptr_base_h_[b];
@@ -356,7 +418,8 @@ int DeviceSet::size_pools_from_counts() {
     M.pool[0] = pool_u_ + pool_base_u_[b];
     M.pool[1] = pool_l_ + pool_base_l_[b];
   }
-  GSOFA_HIP(hipMemcpy(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, hipMemcpyHostToDevice), "upload table");
+  GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
+  if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(offsets)");
   pools_sized_ = true;
   return GSOFA_OK;
 }
@@ -370,8 +433,8 @@ int DeviceSet::read_structure(int mat, std::vector<int64_t>& u_ptr, std::vector<
   const int64_t nu = u_ptr.back(), nl = l_ptr.back();
   u_idx.resize((size_t)nu);
   l_idx.resize((size_t)nl);
-  if (nu) GSOFA_HIP(hipMemcpy(u_idx.data(), pool_u_ + pool_base_u_[mat], (size_t)nu * 4, hipMemcpyDeviceToHost), "read U");
-  if (nl) GSOFA_HIP(hipMemcpy(l_idx.data(), pool_l_ + pool_base_l_[mat], (size_t)nl * 4, hipMemcpyDeviceToHost), "read L");
+  if (nu) GSOFA_TRY(download(u_idx.data(), pool_u_ + pool_base_u_[mat], (size_t)nu * 4, "read U"));
+  if (nl) GSOFA_TRY(download(l_idx.data(), pool_l_ + pool_base_l_[mat], (size_t)nl * 4, "read L"));
   const int32_t n = ns_[mat];
   for (int32_t v = 0; v < n; ++v) {
     std::sort(u_idx.begin() + u_ptr[v], u_idx.begin() + u_ptr[v + 1]);

@@ -35,6 +35,8 @@ struct Options {
   int32_t light_wgs_per_cu = 0; // 0 = tuned
   int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
   int32_t chain_block = 64;     // rows per workgroup-path item; 1 disables chaining
+  int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
+                                // concurrent sets on other streams overlap on the device
 };
 
 struct MatResult {
@@ -56,7 +58,11 @@ int device_info(int device, DeviceInfo& info);
 class DeviceSet {
  public:
   ~DeviceSet();
-  int create(const std::vector<const Prepared*>& mats, const Options& opt);
+  // alloc_stream != null: stream-ordered allocations and uploads on that stream (no device-wide
+  // synchronisation; used by the server), freed on the same stream by the destructor.
+  int create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream = nullptr);
+  // Device bytes create() will allocate for these matrices (admission by footprint).
+  static int64_t footprint(const std::vector<const Prepared*>& mats, const Options& opt, int64_t* out_bytes);
   // Enqueue the whole symbolic factorisation on `s` (no host synchronisation, no allocation,
   // no memset nodes: safe to capture in a graph).
   int enqueue(hipStream_t s, hipEvent_t* stage_events = nullptr);
@@ -117,6 +123,10 @@ class DeviceSet {
   std::vector<std::vector<int64_t>> u_ptr_h_, l_ptr_h_;    // host copies of the offsets
   std::vector<MatDev> mats_h_;
   std::vector<void*> owned_;
+  hipStream_t alloc_stream_ = nullptr;
+  std::vector<std::vector<int32_t>> staging_;  // host copies kept alive for async uploads
+  int upload(void* dst, const void* src, size_t bytes, const char* what);
+  int download(void* dst, const void* src, size_t bytes, const char* what);
 };
 
 }  // namespace gsofa

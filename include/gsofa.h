@@ -168,6 +168,38 @@ int gsofa_batch_nzu_get(gsofa_batch* batch, int32_t index, int32_t* nzu);
 int gsofa_batch_supernodes_get(gsofa_batch* batch, int32_t index, uint8_t* flags);
 void gsofa_batch_destroy(gsofa_batch* batch);
 
+/* ---- serving: continuous batching of requests ------------------------------------------
+ * Requests are queued FIFO and admitted onto one of `slots` device streams when a slot is free
+ * and their device footprint fits the HBM budget; admitted requests run concurrently (grids
+ * sized for cus / slots). A request that can never fit is rejected (GSOFA_ERR_REJECTED from
+ * poll/wait). Results are the same as a plan's on the same pattern. No counterpart in the
+ * reference. */
+typedef struct gsofa_server gsofa_server;
+typedef struct gsofa_server_options {
+  int32_t device;
+  int32_t slots;            /* concurrent requests on the device, default 4                      */
+  int64_t hbm_budget_bytes; /* admission budget; 0 = 80% of the free device memory at start      */
+} gsofa_server_options;
+void gsofa_server_options_default(gsofa_server_options* opt);
+int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out);
+/* Copies and prepares the pattern on the calling thread, then queues it. opt: chunk_size,
+ * n_processes and tuning fields are used; structure is not produced by the server. */
+int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_options* opt, int64_t* request_id);
+/* GSOFA_OK with the result, or GSOFA_ERR_NOT_READY / _CANCELLED / _REJECTED / another error. */
+int gsofa_server_poll(gsofa_server* srv, int64_t request_id, gsofa_result* out);
+/* Blocks up to timeout_s seconds (< 0: no limit). */
+int gsofa_server_wait(gsofa_server* srv, int64_t request_id, double timeout_s, gsofa_result* out);
+/* Pending requests are dropped; running ones finish on the device and report GSOFA_ERR_CANCELLED. */
+int gsofa_server_cancel(gsofa_server* srv, int64_t request_id);
+/* Submit-to-result latency, time queued before admission, and FNV-1a digests of nzU and the
+ * supernode flags of a finished request. */
+int gsofa_server_request_info(gsofa_server* srv, int64_t request_id, double* latency_ms, double* queue_ms,
+                              uint64_t* nzu_digest, uint64_t* supernode_digest);
+int gsofa_server_stats(gsofa_server* srv, int64_t* completed, int32_t* running, int32_t* peak_running,
+                       int64_t* pending, int64_t* hbm_in_use, int64_t* hbm_budget);
+/* Drops pending requests, waits for running ones, releases everything. */
+int gsofa_server_stop(gsofa_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

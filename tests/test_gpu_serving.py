@@ -1,0 +1,83 @@
+"""GPU: the server's results equal a plan's, alone and in a mix of concurrent requests; admission,
+rejection and cancellation."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import gsofatpds_amd as g
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def digest_arrays(nzu, flags):
+    return g.fnv1a(nzu.astype(np.int32).tobytes()), g.fnv1a(flags.astype(np.uint8).tobytes())
+
+
+def plan_reference(p, **opt):
+    with g.Plan(p, **opt) as plan:
+        plan.run()
+        r = plan.result()
+        return r.as_dict(), digest_arrays(plan.nzu(), plan.supernode_flags())
+
+
+KEYS = ("fill_count", "nnz_F", "nnz_L", "nnz_U", "edge_checks", "reach", "supernodes")
+
+
+def test_server_alone_and_in_the_mix():
+    members = [6, 11, 2, 8, 9, 12, 14, 16]
+    pats = [g.generate_workload(g.WL_STREAM, m) if m % 2 == 0 else g.generate_workload(g.WL_BATCH256, m)
+            for m in members]
+    pats = [p for p in pats if p.n <= 60000][:6]
+    say(f"[serve] n = {[p.n for p in pats]}")
+    refs = [plan_reference(p) for p in pats]
+    srv = g.Server(slots=4)
+    try:
+        # alone: one at a time
+        for p, (ref, dig) in zip(pats, refs):
+            rid = srv.submit(p)
+            r = srv.wait(rid, 120.0).as_dict()
+            info = srv.info(rid)
+            assert all(r[k] == ref[k] for k in KEYS)
+            assert (info["nzu_digest"], info["supernode_digest"]) == dig
+        # in the mix: all at once, twice over, so requests queue and overlap
+        ids = [srv.submit(p) for p in pats + pats]
+        for k, rid in enumerate(ids):
+            r = srv.wait(rid, 300.0).as_dict()
+            ref, dig = refs[k % len(pats)]
+            assert all(r[key] == ref[key] for key in KEYS), (k, r, ref)
+            info = srv.info(rid)
+            assert (info["nzu_digest"], info["supernode_digest"]) == dig
+        st = srv.stats()
+        say(f"[serve] stats {st}")
+        assert st["peak_running"] >= 2  # requests really overlapped
+    finally:
+        srv.stop()
+
+
+def test_server_rejects_what_never_fits_and_cancels():
+    small = g.generate_workload(g.WL_GRID2D_4096)
+    srv = g.Server(slots=1, hbm_budget_bytes=64 << 20)
+    try:
+        big = g.generate_workload(g.WL_GRID3D_110592)
+        rid = srv.submit(big)
+        with pytest.raises(g.GsofaError) as e:
+            srv.wait(rid, 10.0)
+        assert e.value.code == g.ERR_REJECTED
+        # queue three; cancel the last while it is still pending
+        a = srv.submit(small)
+        b = srv.submit(small)
+        c = srv.submit(small)
+        srv.cancel(c)
+        ra, rb = srv.wait(a, 60.0), srv.wait(b, 60.0)
+        assert ra.fill_count == rb.fill_count
+        with pytest.raises(g.GsofaError) as e:
+            srv.wait(c, 10.0)
+        assert e.value.code == g.ERR_CANCELLED
+    finally:
+        srv.stop()
