@@ -34,7 +34,90 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_heavy_grid2d_659033.json"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the dominant kernel (optional)")
+    ap.add_argument("--batch-members", type=int, default=0,
+                    help="members of the 256-matrix batch to run batched vs plain (0 = skip)")
+    ap.add_argument("--served-requests", type=int, default=0,
+                    help="requests of the Poisson stream to serve (0 = skip)")
+    ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--slots", type=int, default=4)
     return ap.parse_args()
+
+
+def batched_vs_plain(g, members):
+    """The batch workload: batched (one captured graph for all members) vs one plan per member
+    run back to back. Per-member results must be identical."""
+    pats = [g.generate_workload(g.WL_BATCH256, k) for k in range(members)]
+    log(f"[batched] {members} members, n total {sum(p.n for p in pats)}")
+    b = g.Batch(pats)
+    b.run().sync()  # capture + warm-up
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run().sync()
+    t_batch = (time.perf_counter() - t0) / reps
+    keys = ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes")
+    batch_res = [tuple(getattr(b.result(i), k) for k in keys) for i in range(members)]
+    b.close()
+    t_plain = 0.0
+    mismatches = 0
+    for i, p in enumerate(pats):
+        with g.Plan(p) as plan:
+            plan.run().sync()  # warm-up (allocation and first launch excluded, as for the batch)
+            t0 = time.perf_counter()
+            plan.run().sync()
+            t_plain += time.perf_counter() - t0
+            r = plan.result()
+            mismatches += tuple(getattr(r, k) for k in keys) != batch_res[i]
+    log(f"[batched] batch {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, mismatches {mismatches}")
+    return {"members": members, "batched_matrices_per_s": members / t_batch, "plain_matrices_per_s": members / t_plain,
+            "speedup": t_plain / t_batch, "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
+            "bit_exact_vs_plain": mismatches == 0}
+
+
+def served_vs_plain(g, nreq, rate, slots):
+    """Poisson arrivals (seeded) of stream requests into the continuous-batching server; p50/p99
+    submit-to-result latency and matrices/s, against the same requests run as plain plans one
+    after another. Results must be identical alone and in the mix."""
+    import numpy as np
+    rng = np.random.default_rng(0x5EED0005)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    pats = [g.generate_workload(g.WL_STREAM, k) for k in range(nreq)]
+    log(f"[served] {nreq} requests, n from {min(p.n for p in pats)} to {max(p.n for p in pats)}")
+    keys = ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes")
+    srv = g.Server(slots=slots)
+    ids = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for p, gap in zip(pats, gaps):
+        t_next += gap
+        delay = t_next - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(p))
+    lat, res = [], []
+    for rid in ids:
+        r = srv.wait(rid, 600.0)
+        res.append(tuple(getattr(r, k) for k in keys))
+        lat.append(srv.info(rid)["latency_ms"])
+    t_served = time.perf_counter() - t_start
+    st = srv.stats()
+    srv.stop()
+    t_plain = 0.0
+    mismatches = 0
+    for p, rr in zip(pats, res):
+        with g.Plan(p) as plan:
+            t0 = time.perf_counter()
+            plan.run().sync()
+            t_plain += time.perf_counter() - t0
+            r = plan.result()
+            mismatches += tuple(getattr(r, k) for k in keys) != rr
+    lat = np.array(lat)
+    log(f"[served] wall {t_served:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms, "
+        f"plain {t_plain:.2f} s, peak concurrent {st['peak_running']}")
+    return {"requests": nreq, "rate_per_s": rate, "slots": slots, "matrices_per_s": nreq / t_served,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "plain_sequential_matrices_per_s": nreq / t_plain, "peak_concurrent": st["peak_running"],
+            "identical_alone_and_in_mix": mismatches == 0}
 
 
 def log(msg):
@@ -141,6 +224,10 @@ def main():
         "roofline": roofline,
     }
     plan.close()
+    if args.batch_members > 0:
+        out["batched"] = batched_vs_plain(g, args.batch_members)
+    if args.served_requests > 0:
+        out["served"] = served_vs_plain(g, args.served_requests, args.served_rate, args.slots)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)
