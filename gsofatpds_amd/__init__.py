@@ -90,6 +90,7 @@ _SIGNATURES = {
     "gsofa_device_query": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.c_char_p, C.c_int32]),
     "gsofa_device_synchronize": (C.c_int, [C.c_int]),
+    "gsofa_bandwidth_probe": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "gsofa_stream_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "gsofa_stream_destroy": (C.c_int, [C.c_void_p]),
     "gsofa_options_default": (None, [C.POINTER(Options)]),
@@ -250,6 +251,13 @@ def device_query(device: int = 0) -> dict:
     arch = C.create_string_buffer(128)
     _check(lib().gsofa_device_query(device, C.byref(cus), C.byref(lb), C.byref(lc), arch, 128))
     return {"cus": cus.value, "lds_per_block": lb.value, "lds_per_cu": lc.value, "arch": arch.value.decode()}
+
+
+def bandwidth_probe(nbytes: int = 1 << 30, iters: int = 10, device: int = 0) -> float:
+    """GB/s of a streaming device copy (read + write bytes)."""
+    v = C.c_double()
+    _check(lib().gsofa_bandwidth_probe(device, nbytes, iters, C.byref(v)))
+    return v.value
 
 
 def device_synchronize(device: int = 0) -> None:

@@ -143,6 +143,39 @@ int gsofa_stream_destroy(void* s) {
   return GSOFA_OK;
 }
 
+int gsofa_bandwidth_probe(int device, int64_t bytes, int32_t iters, double* gb_per_s) {
+  if (bytes < (1 << 20) || iters <= 0 || !gb_per_s) return fail(GSOFA_ERR_INVALID_ARG, "bad probe arguments");
+  if (hipSetDevice(device) != hipSuccess) return fail(GSOFA_ERR_NO_DEVICE, "hipSetDevice");
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipGetDeviceProperties");
+  const int64_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, (size_t)n16 * 16) != hipSuccess) return fail(GSOFA_ERR_OUT_OF_MEMORY, "probe buffer");
+  if (hipMalloc(&b, (size_t)n16 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(GSOFA_ERR_OUT_OF_MEMORY, "probe buffer");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int grid = p.multiProcessorCount * 8;
+  int rc = GSOFA_OK;
+  if (gsofa::launch_copy16(a, b, n16, grid, nullptr) != hipSuccess) rc = fail(GSOFA_ERR_HIP, "copy kernel");
+  (void)hipEventRecord(e0, nullptr);
+  for (int i = 0; i < iters && rc == GSOFA_OK; ++i)
+    if (gsofa::launch_copy16(a, b, n16, grid, nullptr) != hipSuccess) rc = fail(GSOFA_ERR_HIP, "copy kernel");
+  (void)hipEventRecord(e1, nullptr);
+  float ms = 0.f;
+  if (rc == GSOFA_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(GSOFA_ERR_HIP, "probe timing");
+  if (rc == GSOFA_OK) *gb_per_s = 2.0 * (double)n16 * 16.0 * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
 void gsofa_options_default(gsofa_options* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));

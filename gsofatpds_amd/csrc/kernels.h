@@ -90,5 +90,6 @@ constexpr int kLightThreads = 256;
 namespace gsofa {
 // Resident workgroups of k_heavy per CU for a block size and dynamic LDS size.
 int heavy_occupancy(int threads, size_t lds_bytes);
+hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipStream_t s);
 int light_occupancy();
 }  // namespace gsofa

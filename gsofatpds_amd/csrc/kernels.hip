@@ -877,6 +877,19 @@ hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_b
 }  // namespace gsofa
 
 namespace gsofa {
+// Streaming copy of 16-byte words: the calibration kernel for HBM byte counters and the
+// achievable-bandwidth probe (gsofa_bandwidth_probe).
+__global__ __launch_bounds__(256) void k_copy16(const v4i32* __restrict__ src, v4i32* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, static_cast<const v4i32*>(src), static_cast<v4i32*>(dst),
+                     n16);
+  return hipGetLastError();
+}
+
 int heavy_occupancy(int threads, size_t lds_bytes) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;

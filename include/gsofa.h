@@ -103,6 +103,11 @@ int gsofa_device_synchronize(int device);
 int gsofa_stream_create(int device, void** hip_stream);
 int gsofa_stream_destroy(void* hip_stream);
 
+/* Streaming device copy of `bytes` (read + write counted), `iters` times: the achievable HBM
+ * bandwidth, and the reference kernel ("k_copy16") that calibrates byte counters in profiles.
+ * No counterpart in the reference. */
+int gsofa_bandwidth_probe(int device, int64_t bytes, int32_t iters, double* gb_per_s);
+
 /* ---- structured API --------------------------------------------------------------------- */
 typedef struct gsofa_options {
   int32_t chunk_size;       /* supernode chunk S (argv[5]); default 128                      */
