@@ -201,7 +201,7 @@ def test_entry_and_cli_on_reference_files(tmp_path):
     assert f"Final Number of Supernodes detected: {o['nsup'] + 1}" in r.stdout
     csv = (tmp_path / "Vary_percent_blocksi_pre2.csv").read_text().splitlines()
     assert csv[0] == "vert_count;N_gpus;p0;p1;#blocks_src;min_time;max_time"
-    assert csv[1].startswith(files[1] + ";1;0;0;0;")
+    assert csv[1].startswith(files[1] + ";1;0;0;1;")  # argv[2];argv[4];argv[6];argv[7];argv[8]
 
 
 def _large(name):
