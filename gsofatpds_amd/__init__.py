@@ -106,6 +106,7 @@ _SIGNATURES = {
     "gsofa_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "gsofa_work_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),
+    "gsofa_executed_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "gsofa_plan_destroy": (None, [C.c_void_p]),
     "gsofa_batch_create": (C.c_int, [C.POINTER(CSR), C.c_int32, C.POINTER(Options), C.POINTER(C.c_void_p)]),
     "gsofa_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -364,6 +365,12 @@ class Plan:
         w, r, e = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(lib().gsofa_work_split(self._h, C.byref(w), C.byref(r), C.byref(e)))
         return {"heavy_edge_checks": w.value, "heavy_reach": r.value, "heavy_entries": e.value}
+
+    def executed_work(self) -> dict:
+        w = (C.c_uint64 * 6)()
+        _check(lib().gsofa_executed_work(self._h, w))
+        return {"heavy_edges": w[0], "heavy_rows": w[1], "heavy_out": w[2],
+                "light_edges": w[3], "light_rows": w[4], "light_out": w[5]}
 
     def structure(self):
         """(u_rowptr, u_col, l_colptr, l_row): strict U by rows, strict L by columns, sorted."""

@@ -325,6 +325,16 @@ int gsofa_work_split(gsofa_plan* p, uint64_t* heavy_edge_checks, uint64_t* heavy
   return GSOFA_OK;
 }
 
+int gsofa_executed_work(gsofa_plan* p, uint64_t* work6) {
+  if (!p || !work6) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  for (int k = 0; k < 6; ++k) work6[k] = p->results[0].exec[k];
+  return GSOFA_OK;
+}
+
 void gsofa_plan_destroy(gsofa_plan* p) { delete p; }
 
 // ---- batches ---------------------------------------------------------------------------------

@@ -350,6 +350,7 @@ int DeviceSet::read_results(std::vector<MatResult>& out) {
     r.heavy_w = a[kAccHeavyW];
     r.heavy_r = a[kAccHeavyR];
     r.heavy_nnz = a[kAccHeavyNnz];
+    for (int k = 0; k < 6; ++k) r.exec[k] = a[kAccHeavyExecEdges + k];
     r.nsup = (int64_t)a[kAccNsup];
   }
   return GSOFA_OK;

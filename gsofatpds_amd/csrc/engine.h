@@ -42,6 +42,7 @@ struct Options {
 struct MatResult {
   uint64_t fill = 0, W = 0, R = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
   uint64_t heavy_w = 0, heavy_r = 0, heavy_nnz = 0;  // the share of the workgroup path
+  uint64_t exec[6] = {0, 0, 0, 0, 0, 0};            // executed edges/rows/outputs: heavy, light
   int64_t nsup = 0;
 };
 

@@ -25,7 +25,15 @@ enum : int {
   kAccHeavyW = 10,   // edge checks of row traversals done on the workgroup path
   kAccHeavyR = 11,   // |R_i| of row traversals done on the workgroup path
   kAccHeavyNnz = 12, // strict F entries found on the workgroup path (both directions)
-  kAccSlots = 16
+  // work the kernels actually executed (both directions; chaining and the wave path's hash
+  // make these differ from the schedule-independent W and |R|)
+  kAccHeavyExecEdges = 13,  // neighbour inspections done by k_heavy
+  kAccHeavyExecVerts = 14,  // rows expanded by k_heavy
+  kAccHeavyExecOut = 15,    // F entries k_heavy wrote (U lists incl. carried entries)
+  kAccLightExecEdges = 16,  // neighbour inspections done by k_light (escalated attempts included)
+  kAccLightExecVerts = 17,  // rows expanded by k_light
+  kAccLightExecOut = 18,    // F entries of the sources k_light finished
+  kAccSlots = 24
 };
 
 // One matrix as the kernels see it. Direction 0 traverses A (rows, gives U), direction 1

@@ -307,6 +307,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
 
   // per-wave running totals, flushed once at exit (one atomic per wave and slot)
   uint64_t t_fill = 0, t_w = 0, t_r = 0, t_nnz_u = 0, t_nnz_l = 0;
+  uint64_t x_edges = 0, x_verts = 0, x_out = 0;  // executed work (per lane)
   int last_mat = -1;
 
   const int64_t gwave = (int64_t)blockIdx.x * kLightWaves + wave;
@@ -316,8 +317,12 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
     if (mb < 0) return;
     const uint64_t f = wave_sum_u64(t_fill), w = wave_sum_u64(t_w), r = wave_sum_u64(t_r);
     const uint64_t nu = wave_sum_u64(t_nnz_u), nl = wave_sum_u64(t_nnz_l);
+    const uint64_t xe = wave_sum_u64(x_edges), xv = wave_sum_u64(x_verts), xo = wave_sum_u64(x_out);
     if (lane == 0) {
       unsigned long long* acc = mats[mb].acc;
+      if (xe) atomicAdd(&acc[kAccLightExecEdges], xe);
+      if (xv) atomicAdd(&acc[kAccLightExecVerts], xv);
+      if (xo) atomicAdd(&acc[kAccLightExecOut], xo);
       if (f) atomicAdd(&acc[kAccFill], f);
       if (w) atomicAdd(&acc[kAccW], w);
       if (r) atomicAdd(&acc[kAccR], r);
@@ -325,6 +330,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
       if (nl) atomicAdd(&acc[kAccNnzL], nl);
     }
     t_fill = t_w = t_r = t_nnz_u = t_nnz_l = 0;
+    x_edges = x_verts = x_out = 0;
   };
 
   for (int64_t k = gwave; k < nitems; k += nwaves) {
@@ -393,6 +399,8 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
         if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
         visit(m);
       }
+      x_edges += deg;
+      x_verts += v >= 0;
       if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
       w_acc += deg;
       if (esc) break;
@@ -422,6 +430,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
     const uint64_t W = wave_sum_u64(w_acc);
     if (lane == 0) {
       M.cnt[dir][src] = (int32_t)U;
+      x_out += U;
       t_fill += U - Useed;
       if (dir == 0) {
         t_w += W;
@@ -473,6 +482,7 @@ __device__ __forceinline__ void qput(int32_t* l, int32_t* g, int32_t qcap, int32
 
 struct Totals {
   uint64_t fill = 0, w = 0, r = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
+  uint64_t x_edges = 0, x_verts = 0, x_out = 0;
 };
 // Every k_heavy total is also the workgroup path's own share: W, R and F entries found there.
 
@@ -488,6 +498,9 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
   if (t.w) atomicAdd(&acc[kAccHeavyW], t.w);
   if (t.r) atomicAdd(&acc[kAccHeavyR], t.r);
   if (t.nnz_u + t.nnz_l) atomicAdd(&acc[kAccHeavyNnz], t.nnz_u + t.nnz_l);
+  if (t.x_edges) atomicAdd(&acc[kAccHeavyExecEdges], t.x_edges);
+  if (t.x_verts) atomicAdd(&acc[kAccHeavyExecVerts], t.x_verts);
+  if (t.x_out) atomicAdd(&acc[kAccHeavyExecOut], t.x_out);
   t = Totals{};
 }
 
@@ -711,6 +724,9 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
         M.cnt[dir][src] = (int32_t)U;
         tot.fill += U - s_useed;
         tot.heavy += 1;
+        tot.x_edges += w_src;  // duplicate surplus included in direction 0, as W counts it
+        tot.x_verts += (uint64_t)r_new + 1;
+        tot.x_out += U;
         if (dir == 0) {
           tot.w += w_src + carried_wr;
           tot.r += (uint64_t)r_new + carried_r;

@@ -160,6 +160,11 @@ int gsofa_stage_times(gsofa_plan* plan, double* ms);
 /* The workgroup path's share of the last run: row-traversal edge checks, |R_i| sum, and strict
  * F entries (both directions). Used to price that kernel alone. */
 int gsofa_work_split(gsofa_plan* plan, uint64_t* heavy_edge_checks, uint64_t* heavy_reach, uint64_t* heavy_entries);
+/* Work the last run actually executed, both directions: work6 = {workgroup path: neighbour
+ * inspections, rows expanded, F entries written; wave path: the same three}. Chaining and the
+ * two directions make these differ from W and |R|; the roofline of each kernel is priced on
+ * them. No counterpart in the reference. */
+int gsofa_executed_work(gsofa_plan* plan, uint64_t* work6);
 void gsofa_plan_destroy(gsofa_plan* plan);
 
 /* ---- batched form: many matrices in one captured graph of persistent launches ----------- */
