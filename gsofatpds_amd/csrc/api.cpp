@@ -25,7 +25,7 @@ Options to_options(const gsofa_options* o) {
   r.heavy_wgs_per_cu = o->heavy_wgs_per_cu;
   r.light_wgs_per_cu = o->light_wgs_per_cu;
   r.qcap = o->queue_capacity;
-  r.chain_block = o->chain_block > 0 ? o->chain_block : 64;
+  r.chain_block = o->chain_block > 0 ? o->chain_block : 0;  // 0: chosen from n and the grid
   return r;
 }
 

@@ -120,7 +120,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (nmats_ <= 0) return fail(GSOFA_ERR_INVALID_ARG, "empty matrix set");
   if (nmats_ > (1 << 20)) return fail(GSOFA_ERR_CAPACITY, "too many matrices in one set");
   if (opt.chunk <= 0) return fail(GSOFA_ERR_INVALID_ARG, "chunk size must be positive");
-  if (opt.chain_block <= 0) return fail(GSOFA_ERR_INVALID_ARG, "chain block must be positive");
+  if (opt.chain_block < 0) return fail(GSOFA_ERR_INVALID_ARG, "chain block must not be negative");
   DeviceInfo info;
   GSOFA_TRY(device_info(opt.device, info));
   GSOFA_HIP(hipSetDevice(opt.device), "hipSetDevice");
@@ -142,10 +142,6 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
   }
-  blkq_base_h_.assign(nmats_ + 1, 0);
-  for (int b = 0; b < nmats_; ++b)
-    blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt.chain_block - 1) / opt.chain_block;
-  nblk_total_ = blkq_base_h_[nmats_];
   nrows_ = row_base_h_[nmats_];
   nnz_ = nnz_base_h_[nmats_];
   nitems_ = item_base_h_[nmats_];
@@ -177,6 +173,16 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
   heavy_grid_ = cus * occ;
+  // Rows per workgroup-path item: long blocks chain more, short blocks keep every workgroup
+  // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
+  if (opt_.chain_block == 0) {
+    int64_t k = (int64_t)n_max_ / std::max<int64_t>(1, 2 * (int64_t)heavy_grid_);
+    opt_.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(64, k));
+  }
+  blkq_base_h_.assign(nmats_ + 1, 0);
+  for (int b = 0; b < nmats_; ++b)
+    blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt_.chain_block - 1) / opt_.chain_block;
+  nblk_total_ = blkq_base_h_[nmats_];
   const int occ_l = light_occupancy();
   if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
   light_grid_ = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
@@ -203,7 +209,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
   GSOFA_TRY(alloc(ws_, 1, "work state"));
   GSOFA_TRY(alloc(heavy_items_, nitems_, "escalation list"));
-  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * 3 * (size_t)n_max_, "heavy spill regions"));
+  // per resident workgroup: two frontier spill queues, the U list, the saved bitmap of the
+  // previous row (import)
+  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (3 * (size_t)n_max_ + ((size_t)n_max_ + 31) / 32),
+                  "heavy spill regions"));
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
   GSOFA_TRY(alloc(blk_flag_, (size_t)(2 * nblk_total_), "block flags"));
@@ -270,7 +279,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.esc[1] = esc_ + nrows_ + rb;
     M.blk_flag[0] = blk_flag_ + blkq_base_h_[b];
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
-    M.chain_block = opt.chain_block;
+    M.chain_block = opt_.chain_block;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");
@@ -289,7 +298,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
     nrows += P->n;
     nnz += (int64_t)P->adj.size();
     nmax = std::max(nmax, P->n);
-    nblk += ((int64_t)P->n + opt.chain_block - 1) / std::max(opt.chain_block, 1);
+    nblk += ((int64_t)P->n + 3) / 4;  // smallest automatic chain block
     dup |= !P->dup_u.empty();
   }
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
@@ -300,7 +309,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * nrows * kEll * 2;          // ELL A, A^T
   b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
-  b += 4 * heavy_grid * 3 * (int64_t)nmax;                // spill regions
+  b += 4 * heavy_grid * (3 * (int64_t)nmax + ((int64_t)nmax + 31) / 32);  // spill regions
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
   b += 1 << 20;                       // tables, accumulators, scan scratch

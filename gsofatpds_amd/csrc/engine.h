@@ -34,7 +34,7 @@ struct Options {
   int32_t heavy_wgs_per_cu = 0; // 0 = tuned
   int32_t light_wgs_per_cu = 0; // 0 = tuned
   int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
-  int32_t chain_block = 64;     // rows per workgroup-path item; 1 disables chaining
+  int32_t chain_block = 0;      // rows per workgroup-path item; 1 disables chaining; 0 = auto
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
 };

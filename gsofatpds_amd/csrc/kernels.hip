@@ -293,7 +293,7 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) {
   }
 }
 
-__global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __restrict__ mats, int nmats,
+__global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_light(const MatDev* __restrict__ mats, int nmats,
                                                             const int64_t* __restrict__ item_base, int64_t nitems,
                                                             WorkState* __restrict__ ws,
                                                             uint64_t* __restrict__ heavy_items) {
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
 
   // per-wave running totals, flushed once at exit (one atomic per wave and slot)
   uint64_t t_fill = 0, t_w = 0, t_r = 0, t_nnz_u = 0, t_nnz_l = 0;
-  uint64_t x_edges = 0, x_verts = 0, x_out = 0;  // executed work (per lane)
+  uint32_t x_edges = 0, x_verts = 0, x_out = 0;  // executed work (per lane; a wave's share fits 32 bits)
   int last_mat = -1;
 
   const int64_t gwave = (int64_t)blockIdx.x * kLightWaves + wave;
@@ -399,8 +399,6 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
         if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
         visit(m);
       }
-      x_edges += deg;
-      x_verts += v >= 0;
       if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
       w_acc += deg;
       if (esc) break;
@@ -413,7 +411,10 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
     }
 
     if (esc) {
+      const uint64_t w_spent = wave_sum_u64(w_acc);
       if (lane == 0) {
+        x_edges += (uint32_t)w_spent;  // the attempt's work counts as executed
+        x_verts += (uint32_t)min(qn, kLightQ) + 1;
         gstore(M.esc[dir] + src, (uint8_t)1);
         const int32_t blk = src / M.chain_block;
         if (atomicOr(&M.blk_flag[dir][blk], 1u) == 0u) {
@@ -430,6 +431,8 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(const MatDev* __rest
     const uint64_t W = wave_sum_u64(w_acc);
     if (lane == 0) {
       M.cnt[dir][src] = (int32_t)U;
+      x_edges += (uint32_t)W;
+      x_verts += (uint32_t)qn + 1;
       x_out += U;
       t_fill += U - Useed;
       if (dir == 0) {
@@ -505,11 +508,16 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
 }
 
 // One workgroup per item = a block of chain_block consecutive rows of one direction; it
-// traverses the block's escalated rows in increasing order. Chaining: when row s holds s-1 and
-// s-1 was the previous row, s-1 is in R_s, so R_{s-1} and everything adjacent to it is in the
-// reached set of s (X_{s-1} is a subset of X_s). The bitmap, the U list (entries > s kept) and
-// the work counters then carry over and only vertices new to s are expanded. Otherwise the
-// bitmap is cleared and s is traversed from scratch. Results are identical either way.
+// traverses the block's escalated rows in increasing order and reuses the previous row's
+// reached set X_{s-1} = {s-1} + R_{s-1} + U_{s-1} whenever s-1 is in R_s, because then
+// R_{s-1}, s-1 and all their neighbours belong to X_s:
+//   * direct chaining: row s holds s-1. The bitmap carries over unchanged and only vertices
+//     new to s are expanded.
+//   * import: otherwise, if the previous row's bitmap was saved, s is traversed from scratch
+//     and, the moment s-1 is reached, the saved X_{s-1} is OR-ed in at the next level
+//     boundary; queued vertices already in X_{s-1} are then dropped (their neighbours are in).
+// W_R(s) = sum of deg over R_s and |R_s| are carried exactly: the previous row's values, plus
+// s-1, plus the vertices expanded here that are not in X_{s-1}.
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ mats,
                                                     WorkState* __restrict__ ws,
@@ -520,11 +528,11 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
   constexpr int kWaves = kThreads / 64;
   __shared__ unsigned long long s_idx;
   __shared__ int32_t s_cnt[3];
-  __shared__ uint32_t s_ust, s_useed;
-  __shared__ uint32_t s_win[4];
-  __shared__ unsigned long long s_w;
+  __shared__ uint32_t s_ust, s_unew, s_useed;
+  __shared__ unsigned long long s_wall, s_wovl, s_xe;
+  __shared__ uint32_t s_rall, s_rovl;
   __shared__ int64_t s_off;
-  __shared__ int32_t s_chain, s_deg0;
+  __shared__ int32_t s_chain, s_deg0, s_import, s_src_was_set, s_save_next;
   __shared__ uint32_t s_scan[kWaves];
 
   const int tid = threadIdx.x;
@@ -533,10 +541,12 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
   uint32_t* bm = dyn;
   int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
   int32_t* const ql1 = ql0 + qcap;
-  int32_t* const region = spill_all + (int64_t)blockIdx.x * 3 * n_max;
+  const int64_t region_len = 3 * n_max + (n_max + 31) / 32;
+  int32_t* const region = spill_all + (int64_t)blockIdx.x * region_len;
   int32_t* const qg0 = region;
   int32_t* const qg1 = region + n_max;
-  int32_t* const ustage = region + 2 * n_max;  // U list of the current source, carried on chaining
+  int32_t* const ustage = region + 2 * n_max;                                  // U list of the current row
+  uint32_t* const gprev = reinterpret_cast<uint32_t*>(region + 3 * n_max);    // saved X_{s-1}
 
   int last_mat = -1;
   Totals tot;  // thread 0 only
@@ -558,15 +568,21 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
     const int32_t r1 = min(n, r0 + M.chain_block);
     const bool want_list = M.list_ptr[dir] != nullptr;
 
-    int32_t prev = -1;                    // previous row traversed in this block
-    uint64_t wr_prev = 0, deg_prev = 0;   // W_R and input degree of prev
-    uint64_t r_prev = 0;                  // |R| of prev
+    int32_t prev = -1;                     // previous row traversed in this block
+    bool prev_saved = false;               // gprev holds X_prev
+    uint64_t wr_prev = 0, deg_prev = 0;    // W_R and input degree of prev
+    uint64_t r_prev = 0;                   // |R_prev|
+    uint32_t u_prev = 0;                   // |U_prev|
     for (int32_t src = r0; src < r1; ++src) {
       if (gload(M.esc[dir] + src) == 0) continue;  // the wave path finished this row
       const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
 
-      // ---- chain test: does row src hold src - 1? (wave 0) ----
-      if (tid == 0) s_chain = 0;
+      // ---- direct chain test: does row src hold src - 1? (wave 0) ----
+      if (tid == 0) {
+        s_chain = 0;
+        s_import = 0;
+        s_save_next = 0;
+      }
       __syncthreads();
       if (prev == src - 1 && wave == 0) {
         const int32_t* row = M.ell[dir] + (int64_t)src * kEll;
@@ -581,42 +597,47 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
       }
       __syncthreads();
       const bool chained = s_chain != 0;
+      const bool has_prev = !chained && prev == src - 1 && prev_saved;
 
-      uint64_t carried_wr = 0, carried_r = 0;
       if (!chained) {
         for (int32_t w = tid; w < bw; w += kThreads) bm[w] = 0u;
-        if (tid == 0) s_ust = 0;
-      } else {
-        carried_wr = wr_prev + deg_prev;
-        carried_r = r_prev + 1;
-        // keep the carried U entries above src, in place (reads of a round finish before its writes)
-        const uint32_t len = s_ust;
-        uint32_t kept = 0;
-        for (uint32_t base = 0; base < len; base += kThreads) {
-          const uint32_t i = base + tid;
-          const int32_t m = i < len ? gload(ustage + i) : -1;
-          const bool keep = m > src;
-          const uint64_t bal = __ballot(keep);
-          if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
-          __syncthreads();
-          uint32_t before = 0, all = 0;
-          for (int w = 0; w < kWaves; ++w) {
-            if (w < wave) before += s_scan[w];
-            all += s_scan[w];
-          }
-          if (keep) gstore(ustage + kept + before + rank_below(bal), m);
-          kept += all;
-          __syncthreads();
+        if (tid == 0) {
+          s_ust = 0;
+          s_src_was_set = 0;
         }
-        if (tid == 0) s_ust = kept;
+      } else {
+        if (tid == 0) s_src_was_set = (bm[src >> 5] >> (src & 31)) & 1u;
+        if (want_list) {
+          // keep the carried U entries above src, in place (reads of a round finish before its writes)
+          const uint32_t len = s_ust;
+          uint32_t kept = 0;
+          for (uint32_t base = 0; base < len; base += kThreads) {
+            const uint32_t i = base + tid;
+            const int32_t m = i < len ? gload(ustage + i) : -1;
+            const bool keep = m > src;
+            const uint64_t bal = __ballot(keep);
+            if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, all = 0;
+            for (int w = 0; w < kWaves; ++w) {
+              if (w < wave) before += s_scan[w];
+              all += s_scan[w];
+            }
+            if (keep) gstore(ustage + kept + before + rank_below(bal), m);
+            kept += all;
+            __syncthreads();
+          }
+          if (tid == 0) s_ust = kept;
+        }
       }
       if (tid == 0) {
         s_cnt[0] = 1;
         s_cnt[1] = 0;
         s_cnt[2] = 0;
+        s_unew = 0;
         s_useed = 0;
-        s_win[0] = s_win[1] = s_win[2] = s_win[3] = 0;
-        s_w = 0;
+        s_wall = s_wovl = s_xe = 0;
+        s_rall = s_rovl = 0;
         s_deg0 = 0;
       }
       __syncthreads();
@@ -626,13 +647,15 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
       }
       __syncthreads();
 
-      uint64_t w_acc = 0;
-      uint32_t u_seed = 0;
-      int64_t r_new = 0;
+      uint64_t w_all = 0, w_ovl = 0, x_e = 0;
+      uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
+      // Whether X_{src-1} was imported: a register, identical in every thread (each thread
+      // takes the same branches below). A shared flag written inside the import block could be
+      // seen by a slower wave before it evaluated the block's condition.
+      bool imported = false;
       for (int level = 0;; ++level) {
         const int32_t ncur = s_cnt[level % 3];
         if (ncur == 0) break;
-        if (level > 0) r_new += ncur;
         if (tid == 0) s_cnt[(level + 2) % 3] = 0;
         const bool odd = (level & 1) != 0;
         const int32_t* const cur_l = odd ? ql1 : ql0;
@@ -643,7 +666,10 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
         const bool seed = level == 0;
         for (int32_t base = 0; base < ncur; base += kThreads) {
           const int32_t e = base + tid;
-          const int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+          int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+          bool in_prev = false;
+          if (v >= 0 && !seed && has_prev) in_prev = (gload(gprev + (v >> 5)) >> (v & 31)) & 1u;
+          if (in_prev && imported) v = -1;  // covered by the imported X_{s-1}
           RowCursor rc;
           load_row(M, dir, v, rc);
           uint32_t deg = 0;
@@ -657,6 +683,10 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
               uint32_t* wp = &bm[m >> 5];
               if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
             }
+            if (isnew && has_prev && m == src - 1) {
+              s_import = 1;  // import X_{src-1} at the next level boundary instead of expanding it
+              isnew = false;
+            }
             const bool is_r = isnew && m < src;
             const bool is_u = isnew && m > src;
             const uint64_t mr = __ballot(is_r);
@@ -667,13 +697,16 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
               basepos = __shfl(basepos, leader, 64);
               if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
             }
-            const uint64_t mu = __ballot(is_u);
-            if (mu) {
-              const int leader = __ffsll((unsigned long long)mu) - 1;
-              uint32_t basepos = 0;
-              if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
-              basepos = __shfl(basepos, leader, 64);
-              if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+            u_new += is_u;
+            if (want_list) {
+              const uint64_t mu = __ballot(is_u);
+              if (mu) {
+                const int leader = __ffsll((unsigned long long)mu) - 1;
+                uint32_t basepos = 0;
+                if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
+                basepos = __shfl(basepos, leader, 64);
+                if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+              }
             }
           };
 #pragma unroll
@@ -685,37 +718,108 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
             if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
             visit(m);
           }
+          x_e += deg;
           if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
-          w_acc += deg;
-          if (seed && tid == 0) s_deg0 = (int32_t)deg;
+          if (seed) {
+            if (tid == 0) s_deg0 = (int32_t)deg;
+          } else if (v >= 0) {
+            w_all += deg;
+            r_all += 1;
+            if (in_prev) {
+              w_ovl += deg;
+              r_ovl += 1;
+            }
+          }
         }
         __syncthreads();
+        // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
+        if (!imported && s_import != 0) {
+          // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
+          for (int32_t w = tid; w - tid < bw; w += kThreads) {
+            uint32_t nb = 0;
+            if (w < bw) {
+              const uint32_t pw = gload(gprev + w), cw = bm[w];
+              nb = pw & ~cw;
+              bm[w] = cw | pw;
+              const int32_t b0 = w * 32;
+              if (b0 + 31 <= src) nb = 0;
+              else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
+            }
+            u_new += __popc(nb);
+            if (want_list) {
+              const uint32_t cnt = __popc(nb);
+              // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
+              uint32_t incl = cnt;
+              for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+              }
+              const uint32_t wave_total = __shfl(incl, 63, 64);
+              uint32_t base = 0;
+              if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
+              base = __shfl(base, 0, 64);
+              uint32_t pos = base + incl - cnt;
+              while (nb) {
+                const int b = __ffs(nb) - 1;
+                nb &= nb - 1;
+                gstore(ustage + pos++, w * 32 + b);
+              }
+            }
+          }
+          imported = true;
+          __syncthreads();
+        }
       }
 
-      // ---- this source's totals ----
+      // ---- this row's totals ----
       {
-        const uint64_t us = wave_sum_u64(u_seed), wa = wave_sum_u64(w_acc);
+        const uint64_t a_w = wave_sum_u64(w_all), a_wo = wave_sum_u64(w_ovl), a_x = wave_sum_u64(x_e);
+        const uint64_t a_r = wave_sum_u64(r_all), a_ro = wave_sum_u64(r_ovl);
+        const uint64_t a_us = wave_sum_u64(u_seed), a_un = wave_sum_u64(u_new);
         if (lane == 0) {
-          if (us) atomicAdd(&s_useed, (uint32_t)us);
-          if (wa) atomicAdd(&s_w, (unsigned long long)wa);
+          if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
+          if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
+          if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
+          if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
+          if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
+          if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
+          if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
         }
       }
-      const uint32_t U = s_ust;  // stable: the level loop ended with a barrier
-      if (dir == 1) {
-        uint32_t win[4] = {0, 0, 0, 0};
-        for (uint32_t i = tid; i < U; i += kThreads) window_bit(win, src, gload(ustage + i), chunk_end);
-        const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
-        const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
-        if (lane == 0) {
-          if (w0) atomicOr(&s_win[0], w0);
-          if (w1) atomicOr(&s_win[1], w1);
-          if (w2) atomicOr(&s_win[2], w2);
-          if (w3) atomicOr(&s_win[3], w3);
+      // next row: does it need the saved bitmap? (escalated, and this row is not in it)
+      if (wave == 0 && src + 1 < r1) {
+        const bool next_esc = gload(M.esc[dir] + src + 1) != 0;
+        bool hit = false;
+        if (next_esc) {
+          const int32_t* row = M.ell[dir] + (int64_t)(src + 1) * kEll;
+          const int32_t e = lane < kEll ? gload(row + lane) : kPad;
+          hit = e == src;
+          if (lane == kEll - 1 && e <= -2) {
+            hit = false;
+            for (int32_t p = -(e + 2), pe = gload(M.rowptr[dir] + src + 2); p < pe; ++p)
+              hit |= gload(M.adj[dir] + p) == src;
+          }
         }
+        const uint64_t bh = __ballot(hit);
+        if (lane == 0 && next_esc && bh == 0) s_save_next = 1;
       }
       __syncthreads();
-      const uint64_t w_src = s_w;
       const uint64_t deg0 = (uint64_t)s_deg0;
+      uint64_t wr, rr;
+      uint32_t U;
+      if (imported) {
+        wr = wr_prev + deg_prev + (s_wall - s_wovl);
+        rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
+        U = s_unew;
+      } else if (chained) {
+        wr = wr_prev + deg_prev + s_wall;
+        rr = r_prev + 1 + (uint64_t)s_rall;
+        U = u_prev - (uint32_t)s_src_was_set + s_unew;
+      } else {
+        wr = s_wall;
+        rr = s_rall;
+        U = s_unew;
+      }
       if (tid == 0) {
         if (mb != last_mat) {
           flush_totals(mats, last_mat, tot);
@@ -724,20 +828,35 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
         M.cnt[dir][src] = (int32_t)U;
         tot.fill += U - s_useed;
         tot.heavy += 1;
-        tot.x_edges += w_src;  // duplicate surplus included in direction 0, as W counts it
-        tot.x_verts += (uint64_t)r_new + 1;
+        tot.x_edges += s_xe;
+        tot.x_verts += (uint64_t)s_rall + 1;
         tot.x_out += U;
         if (dir == 0) {
-          tot.w += w_src + carried_wr;
-          tot.r += (uint64_t)r_new + carried_r;
+          tot.w += deg0 + wr;
+          tot.r += rr;
           tot.nnz_u += U;
         } else {
           tot.nnz_l += U;
-          reinterpret_cast<uint4*>(M.win)[src] = make_uint4(s_win[0], s_win[1], s_win[2], s_win[3]);
+          // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
+          uint32_t win[4];
+          for (int q = 0; q < 4; ++q) {
+            const int32_t lo = src + 1 + 32 * q;  // first row of this window word
+            uint32_t word = 0;
+            if (lo < chunk_end) {
+              const int32_t wi = lo >> 5, sh = lo & 31;
+              const uint32_t a = bm[wi];
+              const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
+              word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
+              const int32_t valid = chunk_end - lo;  // rows of the chunk left
+              if (valid < 32) word &= (1u << valid) - 1u;
+            }
+            win[q] = word;
+          }
+          reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
         }
         if (want_list) {
           int64_t off = gload(M.list_ptr[dir] + src);
-          if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U) {
+          if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U || s_ust != U) {
             atomicOr(&M.acc[kAccOverflow], 1ull);
             off = -1;
           }
@@ -750,9 +869,14 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
         if (off >= 0)
           for (uint32_t i = tid; i < U; i += kThreads) gstore(M.pool[dir] + off + i, gload(ustage + i));
       }
-      wr_prev = carried_wr + (w_src - deg0);
+      const bool save = s_save_next != 0;
+      if (save)
+        for (int32_t w = tid; w < bw; w += kThreads) gstore(gprev + w, bm[w]);
+      prev_saved = save;
+      wr_prev = wr;
       deg_prev = deg0;
-      r_prev = carried_r + (uint64_t)r_new;
+      r_prev = rr;
+      u_prev = U;
       prev = src;
       __syncthreads();
     }

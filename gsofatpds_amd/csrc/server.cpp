@@ -255,7 +255,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   r->opt.device = srv->device;
   r->opt.heavy_threads = o->heavy_threads;
   r->opt.qcap = o->queue_capacity;
-  r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 64;
+  r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 0;
   r->opt.structure = 0;
   DeviceInfo info;
   int rc = device_info(srv->device, info);

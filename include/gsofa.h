@@ -118,7 +118,7 @@ typedef struct gsofa_options {
   int32_t heavy_wgs_per_cu; /* tuning: resident bitmap workgroups per CU, 0 = automatic         */
   int32_t light_wgs_per_cu; /* tuning: resident hash-path workgroups per CU, 0 = automatic      */
   int32_t queue_capacity;   /* tuning: LDS frontier entries per level buffer, 0 = automatic     */
-  int32_t chain_block;      /* tuning: rows per workgroup-path item (1 = no chaining), 0 = 64   */
+  int32_t chain_block;      /* tuning: rows per workgroup-path item (1 = no chaining), 0 = auto */
 } gsofa_options;
 
 typedef struct gsofa_result {
