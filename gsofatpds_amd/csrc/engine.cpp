@@ -138,7 +138,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     n_max_ = std::max(n_max_, P.n);
     row_base_h_[b + 1] = row_base_h_[b] + P.n;
     nnz_base_h_[b + 1] = nnz_base_h_[b] + (int64_t)P.adj.size();
-    item_base_h_[b + 1] = item_base_h_[b] + 2 * (int64_t)P.n;
+    item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)P.n + kLightBlock - 1) / kLightBlock);
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
   }
@@ -208,7 +208,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(row_base_d_, nmats_ + 1, "row base"));
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
   GSOFA_TRY(alloc(ws_, 1, "work state"));
-  GSOFA_TRY(alloc(heavy_items_, nitems_, "escalation list"));
+  GSOFA_TRY(alloc(heavy_items_, 2 * nblk_total_, "escalation list"));
   // per resident workgroup: two frontier spill queues, the U list, the saved bitmap of the
   // previous row (import)
   GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (3 * (size_t)n_max_ + ((size_t)n_max_ + 31) / 32),

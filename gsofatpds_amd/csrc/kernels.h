@@ -93,6 +93,7 @@ hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
+constexpr int kLightBlock = 8;  // consecutive rows per wave-path item
 }  // namespace gsofa
 
 namespace gsofa {

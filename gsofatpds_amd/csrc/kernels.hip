@@ -60,6 +60,13 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
+// A wave-uniform value moved to a scalar register (keeps per-wave counters out of VGPRs).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
   return v;
@@ -293,10 +300,13 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) {
   }
 }
 
-__global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_light(const MatDev* __restrict__ mats, int nmats,
-                                                            const int64_t* __restrict__ item_base, int64_t nitems,
-                                                            WorkState* __restrict__ ws,
-                                                            uint64_t* __restrict__ heavy_items) {
+// One wave per item = a block of kLightBlock consecutive rows of one direction, in increasing
+// order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
+// X_s), and once such a chain escalates, its later rows escalate without an attempt.
+
+__global__ __launch_bounds__(kLightWaves * 64) void k_light(
+    const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems,
+    WorkState* __restrict__ ws, uint64_t* __restrict__ heavy_items) {
   __shared__ int32_t s_hash[kLightWaves][kHash];
   __shared__ int32_t s_q[kLightWaves][kLightQ];
   const int wave = threadIdx.x >> 6;
@@ -305,9 +315,9 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
   int32_t* Q = s_q[wave];
   for (int i = lane; i < kHash; i += 64) H[i] = -1;
 
-  // per-wave running totals, flushed once at exit (one atomic per wave and slot)
+  // per-wave running totals, flushed once per matrix (one atomic per wave and slot)
   uint64_t t_fill = 0, t_w = 0, t_r = 0, t_nnz_u = 0, t_nnz_l = 0;
-  uint32_t x_edges = 0, x_verts = 0, x_out = 0;  // executed work (per lane; a wave's share fits 32 bits)
+  uint64_t x_edges = 0, x_verts = 0, x_out = 0;  // wave-uniform
   int last_mat = -1;
 
   const int64_t gwave = (int64_t)blockIdx.x * kLightWaves + wave;
@@ -315,19 +325,16 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
 
   auto flush = [&](int mb) {
     if (mb < 0) return;
-    const uint64_t f = wave_sum_u64(t_fill), w = wave_sum_u64(t_w), r = wave_sum_u64(t_r);
-    const uint64_t nu = wave_sum_u64(t_nnz_u), nl = wave_sum_u64(t_nnz_l);
-    const uint64_t xe = wave_sum_u64(x_edges), xv = wave_sum_u64(x_verts), xo = wave_sum_u64(x_out);
     if (lane == 0) {
       unsigned long long* acc = mats[mb].acc;
-      if (xe) atomicAdd(&acc[kAccLightExecEdges], xe);
-      if (xv) atomicAdd(&acc[kAccLightExecVerts], xv);
-      if (xo) atomicAdd(&acc[kAccLightExecOut], xo);
-      if (f) atomicAdd(&acc[kAccFill], f);
-      if (w) atomicAdd(&acc[kAccW], w);
-      if (r) atomicAdd(&acc[kAccR], r);
-      if (nu) atomicAdd(&acc[kAccNnzU], nu);
-      if (nl) atomicAdd(&acc[kAccNnzL], nl);
+      if (t_fill) atomicAdd(&acc[kAccFill], t_fill);
+      if (t_w) atomicAdd(&acc[kAccW], t_w);
+      if (t_r) atomicAdd(&acc[kAccR], t_r);
+      if (t_nnz_u) atomicAdd(&acc[kAccNnzU], t_nnz_u);
+      if (t_nnz_l) atomicAdd(&acc[kAccNnzL], t_nnz_l);
+      if (x_edges) atomicAdd(&acc[kAccLightExecEdges], x_edges);
+      if (x_verts) atomicAdd(&acc[kAccLightExecVerts], x_verts);
+      if (x_out) atomicAdd(&acc[kAccLightExecOut], x_out);
     }
     t_fill = t_w = t_r = t_nnz_u = t_nnz_l = 0;
     x_edges = x_verts = x_out = 0;
@@ -342,130 +349,166 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     const MatDev& M = mats[mb];
     const int64_t local = k - item_base[mb];
     const int dir = (int)(local & 1);
-    const int32_t src = M.n - 1 - (int32_t)(local >> 1);
-    const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
+    const int32_t nblk = (M.n + kLightBlock - 1) / kLightBlock;
+    const int32_t blk = nblk - 1 - (int32_t)(local >> 1);  // highest rows first
+    const int32_t s0 = blk * kLightBlock, s1 = min(M.n, s0 + kLightBlock);
 
-    if (lane == 0) H[((uint32_t)src * 0x9E3779B1u) >> (32 - kHashBits)] = src;  // empty table
-    int qn = 0, hcount = 1;
-    bool esc = false;
-    uint32_t u_cnt = 0, u_seed = 0;
-    uint64_t w_acc = 0;
-    uint32_t win[4] = {0, 0, 0, 0};
-
-    // batch 0 is the seed row (lane 0 holds src); later batches take up to 64 queued vertices
-    int head = 0;
-    bool seed = true;
-    while (true) {
-      int32_t v;
-      int batch_end;
-      if (seed) {
-        v = (lane == 0) ? src : -1;
-        batch_end = 0;
-      } else {
-        batch_end = min(head + 64, qn);
-        v = (head + lane < batch_end) ? Q[head + lane] : -1;
-      }
+    // state carried from row s-1 (all wave-uniform; hash H holds X_{s-1} when carry_ok)
+    bool carry_ok = false;     // H holds X_{s-1}
+    bool prev_esc = false;     // s-1 escalated
+    int hcount = 0;            // entries of H
+    uint32_t u_prev = 0;       // |U_{s-1}|
+    uint64_t wr_prev = 0, deg_prev = 0, r_prev = 0;
+    for (int32_t src = s0; src < s1; ++src) {
+      const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
       RowCursor rc;
-      load_row(M, dir, v, rc);
-      uint32_t deg = 0;
-      auto visit = [&](int32_t m) {
-        const bool active = m >= 0;
-        deg += active;
-        const int ins = active ? hash_insert(H, m) : 0;
-        const bool is_r = ins == 1 && m < src;
-        const bool is_u = ins == 1 && m > src;
-        const uint64_t mr = __ballot(is_r);
-        const uint64_t mn = __ballot(ins == 1);
-        if (is_r) {
-          const int pos = qn + (int)rank_below(mr);
-          if (pos < kLightQ) Q[pos] = m;
-        }
-        qn += __popcll(mr);
-        hcount += __popcll(mn);
-        if (is_u) {
-          ++u_cnt;
-          if (seed) ++u_seed;
-          if (dir == 1) window_bit(win, src, m, chunk_end);
-        }
-        if (hcount > kLightLimit || qn >= kLightQ) esc = true;
-      };
-      // ELL part (entries first, padding last), then the CSR tail of rows longer than 16
+      load_row(M, dir, lane == 0 ? src : -1, rc);
+      // does row src hold src-1? (lane 0 owns the row)
+      bool has_prev_in_row = false;
+      if (lane == 0 && src > s0) {
 #pragma unroll
-      for (int j = 0; j < kEll; ++j) {
-        if (!esc && __ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+        for (int j = 0; j < kEll; ++j) has_prev_in_row |= rc.e[j] == src - 1;
+        for (int32_t p = rc.ov_pos; p < rc.ov_end; ++p) has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
-      while (!esc && __ballot(rc.ov_pos < rc.ov_end) != 0) {
-        int32_t m = kPad;
-        if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
-        visit(m);
-      }
-      if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
-      w_acc += deg;
-      if (esc) break;
-      if (seed) {
-        seed = false;
+      const bool chained = __ballot(has_prev_in_row) != 0;
+      bool esc = false;
+      if (chained && prev_esc) {
+        esc = true;  // X_src contains the escalated X_{src-1}
       } else {
-        head = batch_end;
-      }
-      if (head >= qn) break;
-    }
-
-    if (esc) {
-      const uint64_t w_spent = wave_sum_u64(w_acc);
-      if (lane == 0) {
-        x_edges += (uint32_t)w_spent;  // the attempt's work counts as executed
-        x_verts += (uint32_t)min(qn, kLightQ) + 1;
-        gstore(M.esc[dir] + src, (uint8_t)1);
-        const int32_t blk = src / M.chain_block;
-        if (atomicOr(&M.blk_flag[dir][blk], 1u) == 0u) {
-          const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
-          heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)blk);
+        const bool carry = chained && carry_ok;
+        if (!carry) {
+          if (hcount > 0)
+            for (int i = lane; i < kHash; i += 64) H[i] = -1;
+          hcount = 0;
+        }
+        bool src_was_set = false;
+        if (lane == 0) src_was_set = hash_insert(H, src) == 0;
+        src_was_set = __ballot(src_was_set) != 0;
+        if (!src_was_set) ++hcount;
+        int qn = 0;
+        uint32_t u_new = 0, u_seed = 0;
+        uint64_t w_new = 0, deg0 = 0;
+        int head = 0;
+        bool seed = true;
+        while (true) {
+          int batch_end = 0;
+          if (!seed) {
+            batch_end = min(head + 64, qn);
+            const int32_t v = (head + lane < batch_end) ? Q[head + lane] : -1;
+            load_row(M, dir, v, rc);
+          }
+          uint32_t deg = 0;
+          auto visit = [&](int32_t m) {
+            const bool active = m >= 0;
+            deg += active;
+            if (seed && m > src) ++u_seed;
+            const int ins = active ? hash_insert(H, m) : 0;
+            const bool is_r = ins == 1 && m < src;
+            const bool is_u = ins == 1 && m > src;
+            const uint64_t mr = __ballot(is_r);
+            const uint64_t mn = __ballot(ins == 1);
+            if (is_r) {
+              const int pos = qn + (int)rank_below(mr);
+              if (pos < kLightQ) Q[pos] = m;
+            }
+            qn += __popcll(mr);
+            hcount += __popcll(mn);
+            u_new += is_u;
+            if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+          };
+#pragma unroll
+          for (int j = 0; j < kEll; ++j) {
+            if (!esc && __ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
+          }
+          while (!esc && __ballot(rc.ov_pos < rc.ov_end) != 0) {
+            int32_t m = kPad;
+            if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
+            visit(m);
+          }
+          const int32_t vexp = seed ? (lane == 0 ? src : -1) : ((head + lane < batch_end) ? Q[head + lane] : -1);
+          if (vexp >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + vexp);
+          if (seed) deg0 = wave_sum_u64(deg);
+          else w_new += deg;
+          if (esc) break;
+          if (seed) {
+            seed = false;
+          } else {
+            head = batch_end;
+          }
+          if (head >= qn) break;
+        }
+        const uint64_t w_new_all = uniform_u64(wave_sum_u64(w_new));
+        deg0 = uniform_u64(deg0);
+        x_edges += w_new_all + deg0;
+        x_verts += (uint64_t)min(qn, kLightQ) + 1;
+        if (!esc) {
+          const uint32_t U = (carry ? u_prev - (src_was_set ? 1u : 0u) : 0u) +
+                             (uint32_t)uniform_u64(wave_sum_u64(u_new));
+          const uint32_t Useed = (uint32_t)uniform_u64(wave_sum_u64(u_seed));
+          const uint64_t wr = (carry ? wr_prev + deg_prev : 0) + w_new_all;
+          const uint64_t rr = (carry ? r_prev + 1 : 0) + (uint64_t)qn;
+          if (lane == 0) M.cnt[dir][src] = (int32_t)U;
+          x_out += U;
+          t_fill += U - Useed;
+          if (dir == 0) {
+            t_w += deg0 + wr;
+            t_r += rr;
+            t_nnz_u += U;
+          } else {
+            t_nnz_l += U;
+          }
+          if (dir == 1) {
+            uint32_t win[4] = {0, 0, 0, 0};
+            for (int i = lane; i < kHash; i += 64) {
+              const int32_t m = H[i];
+              if (m > src) window_bit(win, src, m, chunk_end);
+            }
+            const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
+            const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
+            if (lane == 0) reinterpret_cast<uint4*>(M.win)[src] = make_uint4(w0, w1, w2, w3);
+          }
+          // structure: the U entries (m > src) are the hash members above src
+          if (M.list_ptr[dir] != nullptr) {
+            const int64_t off = gload(M.list_ptr[dir] + src);
+            const int64_t end = gload(M.list_ptr[dir] + src + 1);
+            if (end - off != (int64_t)U) {
+              if (lane == 0) atomicOr(&M.acc[kAccOverflow], 1ull);
+            } else {
+              int64_t wpos = off;
+              for (int i = lane; i < kHash; i += 64) {
+                const int32_t m = H[i];
+                const bool take = m > src;
+                const uint64_t mt = __ballot(take);
+                if (take) gstore(M.pool[dir] + wpos + rank_below(mt), m);
+                wpos += __popcll(mt);
+              }
+            }
+          }
+          carry_ok = true;
+          prev_esc = false;
+          u_prev = U;
+          wr_prev = wr;
+          deg_prev = deg0;
+          r_prev = rr;
         }
       }
+      if (esc) {
+        if (lane == 0) {
+          gstore(M.esc[dir] + src, (uint8_t)1);
+          const int32_t hb = src / M.chain_block;
+          if (atomicOr(&M.blk_flag[dir][hb], 1u) == 0u) {
+            const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
+            heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)hb);
+          }
+        }
+        carry_ok = false;
+        prev_esc = true;
+      }
+    }
+    if (hcount > 0) {
       for (int i = lane; i < kHash; i += 64) H[i] = -1;
-      continue;
+      hcount = 0;
     }
-
-    const uint32_t U = (uint32_t)wave_sum_u64(u_cnt);
-    const uint32_t Useed = (uint32_t)wave_sum_u64(u_seed);
-    const uint64_t W = wave_sum_u64(w_acc);
-    if (lane == 0) {
-      M.cnt[dir][src] = (int32_t)U;
-      x_edges += (uint32_t)W;
-      x_verts += (uint32_t)qn + 1;
-      x_out += U;
-      t_fill += U - Useed;
-      if (dir == 0) {
-        t_w += W;
-        t_r += (uint64_t)qn;
-        t_nnz_u += U;
-      } else {
-        t_nnz_l += U;
-      }
-    }
-    if (dir == 1) {
-      const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
-      const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
-      if (lane == 0) reinterpret_cast<uint4*>(M.win)[src] = make_uint4(w0, w1, w2, w3);
-    }
-    // structure: the U entries (m > src) are the hash members above src
-    if (M.list_ptr[dir] != nullptr) {
-      const int64_t off = gload(M.list_ptr[dir] + src);
-      const int64_t end = gload(M.list_ptr[dir] + src + 1);
-      if (end - off != (int64_t)U) {
-        if (lane == 0) atomicOr(&M.acc[kAccOverflow], 1ull);
-      } else {
-        int64_t wpos = off;
-        for (int i = lane; i < kHash; i += 64) {
-          const int32_t m = H[i];
-          const bool take = m > src;
-          const uint64_t mt = __ballot(take);
-          if (take) gstore(M.pool[dir] + wpos + rank_below(mt), m);
-          wpos += __popcll(mt);
-        }
-      }
-    }
-    for (int i = lane; i < kHash; i += 64) H[i] = -1;
   }
   flush(last_mat);
 }
