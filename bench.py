@@ -40,6 +40,7 @@ def parse():
                     help="requests of the Poisson stream to serve (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--cu-share", type=int, default=0, help="CUs per served request (0 = all)")
     return ap.parse_args()
 
 
@@ -74,7 +75,7 @@ def batched_vs_plain(g, members):
             "bit_exact_vs_plain": mismatches == 0}
 
 
-def served_vs_plain(g, nreq, rate, slots):
+def served_vs_plain(g, nreq, rate, slots, cu_share=0):
     """Poisson arrivals (seeded) of stream requests into the continuous-batching server; p50/p99
     submit-to-result latency and matrices/s, against the same requests run as plain plans one
     after another. Results must be identical alone and in the mix."""
@@ -84,7 +85,7 @@ def served_vs_plain(g, nreq, rate, slots):
     pats = [g.generate_workload(g.WL_STREAM, k) for k in range(nreq)]
     log(f"[served] {nreq} requests, n from {min(p.n for p in pats)} to {max(p.n for p in pats)}")
     keys = ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes")
-    srv = g.Server(slots=slots)
+    srv = g.Server(slots=slots, cu_share=cu_share)
     ids = []
     t_start = time.perf_counter()
     t_next = t_start
@@ -114,7 +115,8 @@ def served_vs_plain(g, nreq, rate, slots):
     lat = np.array(lat)
     log(f"[served] wall {t_served:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, p99 {np.percentile(lat, 99):.1f} ms, "
         f"plain {t_plain:.2f} s, peak concurrent {st['peak_running']}")
-    return {"requests": nreq, "rate_per_s": rate, "slots": slots, "matrices_per_s": nreq / t_served,
+    return {"requests": nreq, "rate_per_s": rate, "slots": slots, "cu_share": cu_share,
+            "matrices_per_s": nreq / t_served,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "plain_sequential_matrices_per_s": nreq / t_plain, "peak_concurrent": st["peak_running"],
             "identical_alone_and_in_mix": mismatches == 0}
@@ -238,7 +240,7 @@ def main():
     if args.batch_members > 0:
         out["batched"] = batched_vs_plain(g, args.batch_members)
     if args.served_requests > 0:
-        out["served"] = served_vs_plain(g, args.served_requests, args.served_rate, args.slots)
+        out["served"] = served_vs_plain(g, args.served_requests, args.served_rate, args.slots, args.cu_share)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)

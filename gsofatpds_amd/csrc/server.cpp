@@ -3,9 +3,10 @@
 // Requests (one pattern each) are prepared on the submitting thread, queued FIFO, and admitted
 // onto one of `slots` HIP streams as soon as a slot is free and their device footprint fits the
 // HBM budget. Each admitted request gets its own device state, allocated and freed
-// stream-ordered on its slot (hipMallocAsync / hipFreeAsync: no device-wide synchronisation),
-// and persistent grids sized for cus / slots compute units, so requests admitted while others
-// run overlap on the device. A completion thread polls the slots' events, reads each result on
+// stream-ordered on its slot (hipMallocAsync / hipFreeAsync: no device-wide synchronisation).
+// Requests on different slots run concurrently: their workgroups fill the CUs others leave
+// free (small requests launch few workgroups; a large one's tail frees CUs progressively), or,
+// with cu_share set, each request's persistent grids are sized for that many CUs. A completion thread polls the slots' events, reads each result on
 // its own stream and releases the slot. Latency is submit-to-result on the host clock.
 // No counterpart in the reference (it runs one matrix per process invocation).
 #include <hip/hip_runtime_api.h>
@@ -69,6 +70,7 @@ uint64_t fnv1a(const void* data, size_t bytes) {
 
 struct gsofa_server {
   int device = 0;
+  int cu_share = 0;
   int64_t budget = 0;
   std::atomic<int64_t> in_use{0};
   std::vector<Slot> slots;
@@ -228,6 +230,7 @@ int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out) {
   std::unique_ptr<gsofa_server> srv(new (std::nothrow) gsofa_server);
   if (!srv) return fail(GSOFA_ERR_OUT_OF_MEMORY, "server");
   srv->device = opt->device;
+  srv->cu_share = opt->cu_share < 0 ? 0 : opt->cu_share;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipMemGetInfo");
   srv->budget = opt->hbm_budget_bytes > 0 ? opt->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
@@ -260,7 +263,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   DeviceInfo info;
   int rc = device_info(srv->device, info);
   if (rc != GSOFA_OK) return rc;
-  r->opt.cu_share = std::max(1, info.cus / (int)srv->slots.size());
+  r->opt.cu_share = srv->cu_share;  // 0: grids sized for the whole device
   rc = prepare(a, r->prep);
   if (rc != GSOFA_OK) return rc;
   rc = (int)DeviceSet::footprint({&r->prep}, r->opt, &r->footprint);

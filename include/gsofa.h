@@ -180,8 +180,8 @@ void gsofa_batch_destroy(gsofa_batch* batch);
 
 /* ---- serving: continuous batching of requests ------------------------------------------
  * Requests are queued FIFO and admitted onto one of `slots` device streams when a slot is free
- * and their device footprint fits the HBM budget; admitted requests run concurrently (grids
- * sized for cus / slots). A request that can never fit is rejected (GSOFA_ERR_REJECTED from
+ * and their device footprint fits the HBM budget; admitted requests run concurrently on the
+ * device. A request that can never fit is rejected (GSOFA_ERR_REJECTED from
  * poll/wait). Results are the same as a plan's on the same pattern. No counterpart in the
  * reference. */
 typedef struct gsofa_server gsofa_server;
@@ -189,6 +189,7 @@ typedef struct gsofa_server_options {
   int32_t device;
   int32_t slots;            /* concurrent requests on the device, default 4                      */
   int64_t hbm_budget_bytes; /* admission budget; 0 = 80% of the free device memory at start      */
+  int32_t cu_share;         /* CUs each request's persistent grids are sized for; 0 = all        */
 } gsofa_server_options;
 void gsofa_server_options_default(gsofa_server_options* opt);
 int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out);
