@@ -156,7 +156,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : (heavy_threads_ == 1024 ? 1 : 2);
   const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
   const size_t bitmap_bytes = (size_t)bw_max_ * 4;
-  const size_t reserve = 512;  // static LDS of the kernel plus margin
+  const size_t reserve = 2048;  // static LDS of the kernel (1.2 KB) plus margin
   if (bitmap_bytes + reserve + 256 * 8 > info.lds_per_block)
     return fail(GSOFA_ERR_CAPACITY, "n = " + std::to_string(n_max_) + " exceeds the LDS bitmap capacity of the device");
   if (opt.qcap) {

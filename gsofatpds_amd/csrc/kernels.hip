@@ -67,9 +67,40 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  return v;
+// DPP lane moves: VALU operations, no LDS round trip. A lane whose source lies outside its row
+// of 16 (row_shr) reads 0; rows outside kRowMask keep 0 (row_bcast).
+template <int kCtrl, int kRowMask = 0xF, bool kBoundZero = true>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xF, kBoundZero);
+}
+
+// Inclusive prefix sum over the 64 lanes: a Hillis-Steele scan inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8), then lane 15 of each row added into rows 1 and 3 and lane 31 into rows
+// 2 and 3 (row_bcast 15 / 31). Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+  x += dpp_u32<0x111>(x);
+  x += dpp_u32<0x112>(x);
+  x += dpp_u32<0x114>(x);
+  x += dpp_u32<0x118>(x);
+  x += dpp_u32<0x142, 0xA, false>(x);
+  x += dpp_u32<0x143, 0xC, false>(x);
+  return x;
+}
+
+// Sum over the wave, the same in every lane.
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u32(x), 63);
+}
+
+// OR over the wave, the same in every lane (the scan pattern above with OR).
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t x) {
+  x |= dpp_u32<0x111>(x);
+  x |= dpp_u32<0x112>(x);
+  x |= dpp_u32<0x114>(x);
+  x |= dpp_u32<0x118>(x);
+  x |= dpp_u32<0x142, 0xA, false>(x);
+  x |= dpp_u32<0x143, 0xC, false>(x);
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
 __device__ __forceinline__ int find_segment(const int64_t* base, int nseg, int64_t k) {
@@ -287,11 +318,15 @@ constexpr int kLightWaves = 4;
 constexpr int kHashBits = 10;
 constexpr int kHash = 1 << kHashBits;
 constexpr int kLightLimit = 512;             // reached vertices before escalation
-constexpr int kLightQ = kLightLimit + 64;    // R entries never exceed the limit plus one row step
+constexpr int kLightStep = 4;                // ELL slots claimed together between escalation checks
+constexpr int kLightQ = kLightLimit + 64 * kLightStep;  // R entries never exceed the limit plus one step
+static_assert(kLightLimit + 64 * kLightStep < kHash, "the hash set must never fill");
 
-__device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) {
-  uint32_t h = ((uint32_t)m * 0x9E3779B1u) >> (32 - kHashBits);
-  // The table is at most (kLightLimit + 64) / kHash full, so an empty slot is always found.
+__device__ __forceinline__ uint32_t hash_home(int32_t m) { return ((uint32_t)m * 0x9E3779B1u) >> (32 - kHashBits); }
+
+// Linear probing from slot h. The table holds at most kLightLimit + 64 * kLightStep < kHash
+// entries, so an empty slot is always found.
+__device__ __forceinline__ int hash_probe(int32_t* H, int32_t m, uint32_t h) {
   while (true) {
     const int32_t old = atomicCAS(&H[h], -1, m);
     if (old == -1) return 1;
@@ -299,6 +334,7 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) {
     h = (h + 1) & (kHash - 1);
   }
 }
+__device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_probe(H, m, hash_home(m)); }
 
 // One wave per item = a block of kLightBlock consecutive rows of one direction, in increasing
 // order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
@@ -398,6 +434,51 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
             load_row(M, dir, v, rc);
           }
           uint32_t deg = 0;
+          // kLightStep slots at a time: their home-slot CAS operations are in flight together;
+          // a collision probes on. New entries below src are queued with one wave scan.
+          auto claim = [&](int32_t m, int32_t old, uint32_t home) -> int {
+            if (old == -1) return 1;
+            if (old == m) return 0;
+            return hash_probe(H, m, (home + 1) & (kHash - 1));
+          };
+#pragma unroll
+          for (int h = 0; h < kEll; h += kLightStep) {
+            if (esc || __ballot(rc.e[h] >= 0) == 0) break;  // rows are packed: no later slot is used either
+            int32_t old[kLightStep];
+            uint32_t home[kLightStep];
+#pragma unroll
+            for (int j = 0; j < kLightStep; ++j) {
+              const int32_t m = rc.e[h + j];
+              home[j] = hash_home(m);
+              old[j] = m >= 0 ? atomicCAS(&H[home[j]], -1, m) : m;
+            }
+            uint32_t mr = 0, nnew = 0;
+#pragma unroll
+            for (int j = 0; j < kLightStep; ++j) {
+              const int32_t m = rc.e[h + j];
+              if (m >= 0) {
+                ++deg;
+                if (seed && m > src) ++u_seed;
+                if (claim(m, old[j], home[j])) {
+                  ++nnew;
+                  if (m < src) mr |= 1u << j;
+                  else u_new += m > src;
+                }
+              }
+            }
+            const uint32_t c = __popc(mr);
+            const uint32_t incl = wave_incl_scan_u32(c);
+            int pos = qn + (int)(incl - c);
+#pragma unroll
+            for (int j = 0; j < kLightStep; ++j)
+              if ((mr >> j) & 1u) {
+                if (pos < kLightQ) Q[pos] = rc.e[h + j];
+                ++pos;
+              }
+            qn += __builtin_amdgcn_readlane((int)incl, 63);
+            hcount += (int)wave_sum_u32(nnew);
+            if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+          }
           auto visit = [&](int32_t m) {
             const bool active = m >= 0;
             deg += active;
@@ -416,10 +497,6 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
             u_new += is_u;
             if (hcount > kLightLimit || qn >= kLightQ) esc = true;
           };
-#pragma unroll
-          for (int j = 0; j < kEll; ++j) {
-            if (!esc && __ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
-          }
           while (!esc && __ballot(rc.ov_pos < rc.ov_end) != 0) {
             int32_t m = kPad;
             if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
@@ -427,7 +504,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
           }
           const int32_t vexp = seed ? (lane == 0 ? src : -1) : ((head + lane < batch_end) ? Q[head + lane] : -1);
           if (vexp >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + vexp);
-          if (seed) deg0 = wave_sum_u64(deg);
+          if (seed) deg0 = wave_sum_u32(deg);
           else w_new += deg;
           if (esc) break;
           if (seed) {
@@ -437,14 +514,15 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
           }
           if (head >= qn) break;
         }
-        const uint64_t w_new_all = uniform_u64(wave_sum_u64(w_new));
+        // per-row sums stay below the input nnz < 2^31
+        const uint64_t w_new_all = wave_sum_u32((uint32_t)w_new);
         deg0 = uniform_u64(deg0);
         x_edges += w_new_all + deg0;
         x_verts += (uint64_t)min(qn, kLightQ) + 1;
         if (!esc) {
           const uint32_t U = (carry ? u_prev - (src_was_set ? 1u : 0u) : 0u) +
-                             (uint32_t)uniform_u64(wave_sum_u64(u_new));
-          const uint32_t Useed = (uint32_t)uniform_u64(wave_sum_u64(u_seed));
+                             wave_sum_u32(u_new);
+          const uint32_t Useed = wave_sum_u32(u_seed);
           const uint64_t wr = (carry ? wr_prev + deg_prev : 0) + w_new_all;
           const uint64_t rr = (carry ? r_prev + 1 : 0) + (uint64_t)qn;
           if (lane == 0) M.cnt[dir][src] = (int32_t)U;
@@ -561,22 +639,34 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
 //     boundary; queued vertices already in X_{s-1} are then dropped (their neighbours are in).
 // W_R(s) = sum of deg over R_s and |R_s| are carried exactly: the previous row's values, plus
 // s-1, plus the vertices expanded here that are not in X_{s-1}.
+//
+// The kernel is bound by dependent global-memory round trips, one workgroup per CU: the
+// per-row facts (escalated? holds its predecessor? structure offsets) are read for a window
+// of up to kHeavyWin rows in one round into LDS, loads of one expansion step are issued
+// together, and the import reads the saved bitmap kImportBatch words per thread per round.
+constexpr int kHeavyWin = 64;
+constexpr int kImportBatch = 8;
+
 template <int kThreads>
-__global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ mats,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_heavy(const MatDev* __restrict__ mats,
                                                     WorkState* __restrict__ ws,
                                                     const uint64_t* __restrict__ heavy_items,
                                                     int32_t bw_max, int32_t qcap, int64_t n_max,
                                                     int32_t* __restrict__ spill_all) {
+  static_assert(kEll == 16, "the window scan maps one row to 16 lanes");
   extern __shared__ uint32_t dyn[];
   constexpr int kWaves = kThreads / 64;
+  constexpr int kPfRounds = ((kHeavyWin + 1) * kEll + kThreads - 1) / kThreads;
   __shared__ unsigned long long s_idx;
   __shared__ int32_t s_cnt[3];
   __shared__ uint32_t s_ust, s_unew, s_useed;
   __shared__ unsigned long long s_wall, s_wovl, s_xe;
   __shared__ uint32_t s_rall, s_rovl;
-  __shared__ int64_t s_off;
-  __shared__ int32_t s_chain, s_deg0, s_import, s_src_was_set, s_save_next;
+  __shared__ int32_t s_deg0, s_import, s_src_was_set;
   __shared__ uint32_t s_scan[kWaves];
+  __shared__ int32_t s_esc[kHeavyWin + 1];   // row w0 + i was escalated
+  __shared__ int32_t s_hold[kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
+  __shared__ int64_t s_lp[kHeavyWin + 2];    // structure offsets of rows w0 .. w0 + np
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
@@ -616,312 +706,385 @@ __global__ __launch_bounds__(kThreads) void k_heavy(const MatDev* __restrict__ m
     uint64_t wr_prev = 0, deg_prev = 0;    // W_R and input degree of prev
     uint64_t r_prev = 0;                   // |R_prev|
     uint32_t u_prev = 0;                   // |U_prev|
-    for (int32_t src = r0; src < r1; ++src) {
-      if (gload(M.esc[dir] + src) == 0) continue;  // the wave path finished this row
-      const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
-
-      // ---- direct chain test: does row src hold src - 1? (wave 0) ----
-      if (tid == 0) {
-        s_chain = 0;
-        s_import = 0;
-        s_save_next = 0;
-      }
-      __syncthreads();
-      if (prev == src - 1 && wave == 0) {
-        const int32_t* row = M.ell[dir] + (int64_t)src * kEll;
-        int32_t e = lane < kEll ? gload(row + lane) : kPad;
-        bool hit = e == src - 1;
-        if (lane == kEll - 1 && e <= -2) {  // long row: scan its CSR tail
-          hit = false;
-          for (int32_t p = -(e + 2), pe = gload(M.rowptr[dir] + src + 1); p < pe; ++p)
-            hit |= gload(M.adj[dir] + p) == src - 1;
-        }
-        if (__ballot(hit) != 0 && lane == 0) s_chain = 1;
-      }
-      __syncthreads();
-      const bool chained = s_chain != 0;
-      const bool has_prev = !chained && prev == src - 1 && prev_saved;
-
-      if (!chained) {
-        for (int32_t w = tid; w < bw; w += kThreads) bm[w] = 0u;
-        if (tid == 0) {
-          s_ust = 0;
-          s_src_was_set = 0;
-        }
-      } else {
-        if (tid == 0) s_src_was_set = (bm[src >> 5] >> (src & 31)) & 1u;
-        if (want_list) {
-          // keep the carried U entries above src, in place (reads of a round finish before its writes)
-          const uint32_t len = s_ust;
-          uint32_t kept = 0;
-          for (uint32_t base = 0; base < len; base += kThreads) {
-            const uint32_t i = base + tid;
-            const int32_t m = i < len ? gload(ustage + i) : -1;
-            const bool keep = m > src;
-            const uint64_t bal = __ballot(keep);
-            if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
-            __syncthreads();
-            uint32_t before = 0, all = 0;
-            for (int w = 0; w < kWaves; ++w) {
-              if (w < wave) before += s_scan[w];
-              all += s_scan[w];
-            }
-            if (keep) gstore(ustage + kept + before + rank_below(bal), m);
-            kept += all;
-            __syncthreads();
-          }
-          if (tid == 0) s_ust = kept;
-        }
-      }
-      if (tid == 0) {
-        s_cnt[0] = 1;
-        s_cnt[1] = 0;
-        s_cnt[2] = 0;
-        s_unew = 0;
-        s_useed = 0;
-        s_wall = s_wovl = s_xe = 0;
-        s_rall = s_rovl = 0;
-        s_deg0 = 0;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        atomicOr(&bm[src >> 5], 1u << (src & 31));
-        ql0[0] = src;
-      }
-      __syncthreads();
-
-      uint64_t w_all = 0, w_ovl = 0, x_e = 0;
-      uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
-      // Whether X_{src-1} was imported: a register, identical in every thread (each thread
-      // takes the same branches below). A shared flag written inside the import block could be
-      // seen by a slower wave before it evaluated the block's condition.
-      bool imported = false;
-      for (int level = 0;; ++level) {
-        const int32_t ncur = s_cnt[level % 3];
-        if (ncur == 0) break;
-        if (tid == 0) s_cnt[(level + 2) % 3] = 0;
-        const bool odd = (level & 1) != 0;
-        const int32_t* const cur_l = odd ? ql1 : ql0;
-        const int32_t* const cur_g = odd ? qg1 : qg0;
-        int32_t* const nxt_l = odd ? ql0 : ql1;
-        int32_t* const nxt_g = odd ? qg0 : qg1;
-        int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
-        const bool seed = level == 0;
-        for (int32_t base = 0; base < ncur; base += kThreads) {
-          const int32_t e = base + tid;
-          int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
-          bool in_prev = false;
-          if (v >= 0 && !seed && has_prev) in_prev = (gload(gprev + (v >> 5)) >> (v & 31)) & 1u;
-          if (in_prev && imported) v = -1;  // covered by the imported X_{s-1}
-          RowCursor rc;
-          load_row(M, dir, v, rc);
-          uint32_t deg = 0;
-          auto visit = [&](int32_t m) {
-            const bool active = m >= 0;
-            deg += active;
-            if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
-            bool isnew = false;
-            if (active) {
-              const uint32_t bit = 1u << (m & 31);
-              uint32_t* wp = &bm[m >> 5];
-              if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
-            }
-            if (isnew && has_prev && m == src - 1) {
-              s_import = 1;  // import X_{src-1} at the next level boundary instead of expanding it
-              isnew = false;
-            }
-            const bool is_r = isnew && m < src;
-            const bool is_u = isnew && m > src;
-            const uint64_t mr = __ballot(is_r);
-            if (mr) {
-              const int leader = __ffsll((unsigned long long)mr) - 1;
-              int32_t basepos = 0;
-              if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
-              basepos = __shfl(basepos, leader, 64);
-              if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
-            }
-            u_new += is_u;
-            if (want_list) {
-              const uint64_t mu = __ballot(is_u);
-              if (mu) {
-                const int leader = __ffsll((unsigned long long)mu) - 1;
-                uint32_t basepos = 0;
-                if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
-                basepos = __shfl(basepos, leader, 64);
-                if (is_u) gstore(ustage + basepos + rank_below(mu), m);
-              }
-            }
-          };
+    for (int32_t w0 = r0; w0 < r1; w0 += kHeavyWin) {
+      const int32_t we = min(r1, w0 + kHeavyWin);
+      const int32_t np = min(r1, we + 1) - w0;  // rows read: the window plus one look-ahead row
+      // ---- window facts, one round of loads ----
+      int32_t esc_v = 0;
+      int64_t lp_v = 0;
+      if (tid < np) esc_v = gload(M.esc[dir] + w0 + tid);
+      if (want_list && tid <= np) lp_v = gload(M.list_ptr[dir] + w0 + tid);
+      int32_t ev[kPfRounds];
 #pragma unroll
-          for (int j = 0; j < kEll; ++j) {
-            if (__ballot(rc.e[j] >= 0) != 0) visit(rc.e[j]);
-          }
-          while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
-            int32_t m = kPad;
-            if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
-            visit(m);
-          }
-          x_e += deg;
-          if (v >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + v);
-          if (seed) {
-            if (tid == 0) s_deg0 = (int32_t)deg;
-          } else if (v >= 0) {
-            w_all += deg;
-            r_all += 1;
-            if (in_prev) {
-              w_ovl += deg;
-              r_ovl += 1;
-            }
-          }
-        }
-        __syncthreads();
-        // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
-        if (!imported && s_import != 0) {
-          // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
-          for (int32_t w = tid; w - tid < bw; w += kThreads) {
-            uint32_t nb = 0;
-            if (w < bw) {
-              const uint32_t pw = gload(gprev + w), cw = bm[w];
-              nb = pw & ~cw;
-              bm[w] = cw | pw;
-              const int32_t b0 = w * 32;
-              if (b0 + 31 <= src) nb = 0;
-              else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
-            }
-            u_new += __popc(nb);
-            if (want_list) {
-              const uint32_t cnt = __popc(nb);
-              // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
-              uint32_t incl = cnt;
-              for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-              }
-              const uint32_t wave_total = __shfl(incl, 63, 64);
-              uint32_t base = 0;
-              if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
-              base = __shfl(base, 0, 64);
-              uint32_t pos = base + incl - cnt;
-              while (nb) {
-                const int b = __ffs(nb) - 1;
-                nb &= nb - 1;
-                gstore(ustage + pos++, w * 32 + b);
-              }
-            }
-          }
-          imported = true;
-          __syncthreads();
-        }
+      for (int q = 0; q < kPfRounds; ++q) {
+        const int32_t i = q * kThreads + tid;
+        ev[q] = i < np * kEll ? gload(M.ell[dir] + (int64_t)w0 * kEll + i) : kPad;
       }
+#pragma unroll
+      for (int q = 0; q < kPfRounds; ++q) {
+        const int32_t i = q * kThreads + tid;
+        const int32_t row = w0 + (i >> 4);
+        bool hit = row > 0 && ev[q] == row - 1;
+        if ((i & (kEll - 1)) == kEll - 1 && ev[q] <= -2 && i < np * kEll) {  // long row: its CSR tail
+          for (int32_t p = -(ev[q] + 2), pe = gload(M.rowptr[dir] + row + 1); p < pe; ++p)
+            hit |= gload(M.adj[dir] + p) == row - 1;
+        }
+        const uint64_t b = __ballot(hit);
+        if ((i & (kEll - 1)) == 0 && i < np * kEll) s_hold[i >> 4] = ((b >> (lane & 48)) & 0xFFFFull) != 0;
+      }
+      if (tid < np) s_esc[tid] = esc_v;
+      if (want_list && tid <= np) s_lp[tid] = lp_v;
+      __syncthreads();
 
-      // ---- this row's totals ----
-      {
-        const uint64_t a_w = wave_sum_u64(w_all), a_wo = wave_sum_u64(w_ovl), a_x = wave_sum_u64(x_e);
-        const uint64_t a_r = wave_sum_u64(r_all), a_ro = wave_sum_u64(r_ovl);
-        const uint64_t a_us = wave_sum_u64(u_seed), a_un = wave_sum_u64(u_new);
-        if (lane == 0) {
-          if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
-          if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
-          if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
-          if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
-          if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
-          if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
-          if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
-        }
-      }
-      // next row: does it need the saved bitmap? (escalated, and this row is not in it)
-      if (wave == 0 && src + 1 < r1) {
-        const bool next_esc = gload(M.esc[dir] + src + 1) != 0;
-        bool hit = false;
-        if (next_esc) {
-          const int32_t* row = M.ell[dir] + (int64_t)(src + 1) * kEll;
-          const int32_t e = lane < kEll ? gload(row + lane) : kPad;
-          hit = e == src;
-          if (lane == kEll - 1 && e <= -2) {
-            hit = false;
-            for (int32_t p = -(e + 2), pe = gload(M.rowptr[dir] + src + 2); p < pe; ++p)
-              hit |= gload(M.adj[dir] + p) == src;
+      for (int32_t src = w0; src < we; ++src) {
+        const int32_t li = src - w0;
+        if (s_esc[li] == 0) continue;  // the wave path finished this row
+        const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
+        const bool follows = prev >= 0 && prev == src - 1;
+        const bool chained = follows && s_hold[li] != 0;
+        const bool has_prev = !chained && follows && prev_saved;
+
+        // ---- start: bitmap, staged U list, counters (one barrier) ----
+        if (!chained) {
+          const int32_t sw = src >> 5;
+          const uint32_t sb = 1u << (src & 31);
+          for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
+          if (tid == 0) {
+            s_ust = 0;
+            s_src_was_set = 0;
           }
-        }
-        const uint64_t bh = __ballot(hit);
-        if (lane == 0 && next_esc && bh == 0) s_save_next = 1;
-      }
-      __syncthreads();
-      const uint64_t deg0 = (uint64_t)s_deg0;
-      uint64_t wr, rr;
-      uint32_t U;
-      if (imported) {
-        wr = wr_prev + deg_prev + (s_wall - s_wovl);
-        rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
-        U = s_unew;
-      } else if (chained) {
-        wr = wr_prev + deg_prev + s_wall;
-        rr = r_prev + 1 + (uint64_t)s_rall;
-        U = u_prev - (uint32_t)s_src_was_set + s_unew;
-      } else {
-        wr = s_wall;
-        rr = s_rall;
-        U = s_unew;
-      }
-      if (tid == 0) {
-        if (mb != last_mat) {
-          flush_totals(mats, last_mat, tot);
-          last_mat = mb;
-        }
-        M.cnt[dir][src] = (int32_t)U;
-        tot.fill += U - s_useed;
-        tot.heavy += 1;
-        tot.x_edges += s_xe;
-        tot.x_verts += (uint64_t)s_rall + 1;
-        tot.x_out += U;
-        if (dir == 0) {
-          tot.w += deg0 + wr;
-          tot.r += rr;
-          tot.nnz_u += U;
         } else {
-          tot.nnz_l += U;
-          // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
-          uint32_t win[4];
-          for (int q = 0; q < 4; ++q) {
-            const int32_t lo = src + 1 + 32 * q;  // first row of this window word
-            uint32_t word = 0;
-            if (lo < chunk_end) {
-              const int32_t wi = lo >> 5, sh = lo & 31;
-              const uint32_t a = bm[wi];
-              const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
-              word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
-              const int32_t valid = chunk_end - lo;  // rows of the chunk left
-              if (valid < 32) word &= (1u << valid) - 1u;
+          if (want_list) {
+            // keep the carried U entries above src, in place (reads of a round finish before its writes)
+            const uint32_t len = s_ust;
+            uint32_t kept = 0;
+            for (uint32_t base = 0; base < len; base += kThreads) {
+              const uint32_t i = base + tid;
+              const int32_t m = i < len ? gload(ustage + i) : -1;
+              const bool keep = m > src;
+              const uint64_t bal = __ballot(keep);
+              if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+              __syncthreads();
+              uint32_t before = 0, all = 0;
+              for (int w = 0; w < kWaves; ++w) {
+                if (w < wave) before += s_scan[w];
+                all += s_scan[w];
+              }
+              if (keep) gstore(ustage + kept + before + rank_below(bal), m);
+              kept += all;
+              __syncthreads();
             }
-            win[q] = word;
+            if (tid == 0) s_ust = kept;
           }
-          reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
-        }
-        if (want_list) {
-          int64_t off = gload(M.list_ptr[dir] + src);
-          if (gload(M.list_ptr[dir] + src + 1) - off != (int64_t)U || s_ust != U) {
-            atomicOr(&M.acc[kAccOverflow], 1ull);
-            off = -1;
+          if (tid == 0) {
+            const uint32_t sb = 1u << (src & 31);
+            const uint32_t old = bm[src >> 5];
+            s_src_was_set = (old & sb) != 0;
+            bm[src >> 5] = old | sb;
           }
-          s_off = off;
         }
-      }
-      if (want_list) {
+        if (tid == 0) {
+          ql0[0] = src;
+          s_cnt[0] = 1;
+          s_cnt[1] = 0;
+          s_cnt[2] = 0;
+          s_unew = 0;
+          s_useed = 0;
+          s_wall = s_wovl = s_xe = 0;
+          s_rall = s_rovl = 0;
+          s_deg0 = 0;
+          s_import = 0;
+        }
         __syncthreads();
-        const int64_t off = s_off;
-        if (off >= 0)
-          for (uint32_t i = tid; i < U; i += kThreads) gstore(M.pool[dir] + off + i, gload(ustage + i));
+
+        // this row's sums (each vertex is expanded at most once per row, so every sum stays
+        // below the input nnz < 2^31)
+        uint32_t w_all = 0, w_ovl = 0, x_e = 0;
+        uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
+        // Whether X_{src-1} was imported: a register, identical in every thread (each thread
+        // takes the same branches below). A shared flag written inside the import block could be
+        // seen by a slower wave before it evaluated the block's condition.
+        bool imported = false;
+        for (int level = 0;; ++level) {
+          const int32_t ncur = s_cnt[level % 3];
+          if (ncur == 0) break;
+          if (tid == 0) s_cnt[(level + 2) % 3] = 0;
+          const bool odd = (level & 1) != 0;
+          const int32_t* const cur_l = odd ? ql1 : ql0;
+          const int32_t* const cur_g = odd ? qg1 : qg0;
+          int32_t* const nxt_l = odd ? ql0 : ql1;
+          int32_t* const nxt_g = odd ? qg0 : qg1;
+          int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
+          const bool seed = level == 0;
+          for (int32_t base = 0; base < ncur; base += kThreads) {
+            const int32_t e = base + tid;
+            int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+            // issued together: membership in the saved X_{s-1}, the row, its duplicate surplus
+            const bool chk = v >= 0 && !seed && has_prev;
+            const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
+            RowCursor rc;
+            load_row(M, dir, v, rc);
+            const uint32_t dupv = (v >= 0 && dir == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
+            const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
+            if (in_prev && imported) {  // covered by the imported X_{s-1}
+              v = -1;
+#pragma unroll
+              for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
+              rc.ov_pos = rc.ov_end = 0;
+            }
+            uint32_t deg = 0;
+            // Claim the bits of 8 slots at a time: the LDS atomics are independent and in flight
+            // together (measured faster than a plain read first). Rows are packed, so the second
+            // group is skipped when no lane has a ninth entry. New entries below src go to the
+            // next level, new entries above src are U.
+            uint32_t mask_r = 0, mask_u = 0;
+#pragma unroll
+            for (int h = 0; h < kEll; h += 8) {
+              if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
+              uint32_t old[8];
+#pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                const int32_t m = rc.e[h + j];
+                old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
+              }
+#pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                const int32_t m = rc.e[h + j];
+                if (m >= 0) {
+                  ++deg;
+                  if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
+                  if (((old[j] >> (m & 31)) & 1u) == 0) {
+                    if (has_prev && m == src - 1) s_import = 1;  // import X_{src-1} instead of expanding it
+                    else if (m < src) mask_r |= 1u << (h + j);
+                    else if (m > src) mask_u |= 1u << (h + j);
+                  }
+                }
+              }
+            }
+            // One queue reservation per wave (exclusive scan of the per-lane counts).
+            {
+              const uint32_t c = __popc(mask_r);
+              const uint32_t incl = wave_incl_scan_u32(c);
+              const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+              if (tot_w != 0) {
+                int32_t basepos = 0;
+                if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
+                int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
+#pragma unroll
+                for (int j = 0; j < kEll; ++j)
+                  if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
+              }
+            }
+            u_new += __popc(mask_u);
+            if (want_list) {
+              const uint32_t c = __popc(mask_u);
+              const uint32_t incl = wave_incl_scan_u32(c);
+              const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+              if (tot_w != 0) {
+                uint32_t basepos = 0;
+                if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
+                uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
+#pragma unroll
+                for (int j = 0; j < kEll; ++j)
+                  if ((mask_u >> j) & 1u) gstore(ustage + pos++, rc.e[j]);
+              }
+            }
+            // the CSR tail of a long row, one entry per lane per step
+            auto visit = [&](int32_t m) {
+              const bool active = m >= 0;
+              deg += active;
+              if (seed && m > src) ++u_seed;
+              bool isnew = false;
+              if (active) {
+                const uint32_t bit = 1u << (m & 31);
+                uint32_t* wp = &bm[m >> 5];
+                if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+              }
+              if (isnew && has_prev && m == src - 1) {
+                s_import = 1;
+                isnew = false;
+              }
+              const bool is_r = isnew && m < src;
+              const bool is_u = isnew && m > src;
+              const uint64_t mr = __ballot(is_r);
+              if (mr) {
+                const int leader = __ffsll((unsigned long long)mr) - 1;
+                int32_t basepos = 0;
+                if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
+                basepos = __shfl(basepos, leader, 64);
+                if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
+              }
+              u_new += is_u;
+              if (want_list) {
+                const uint64_t mu = __ballot(is_u);
+                if (mu) {
+                  const int leader = __ffsll((unsigned long long)mu) - 1;
+                  uint32_t basepos = 0;
+                  if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
+                  basepos = __shfl(basepos, leader, 64);
+                  if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+                }
+              }
+            };
+            while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+              int32_t m = kPad;
+              if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
+              visit(m);
+            }
+            x_e += deg;
+            if (v >= 0) deg += dupv;
+            if (seed) {
+              if (tid == 0) s_deg0 = (int32_t)deg;
+            } else if (v >= 0) {
+              w_all += deg;
+              r_all += 1;
+              if (in_prev) {
+                w_ovl += deg;
+                r_ovl += 1;
+              }
+            }
+          }
+          __syncthreads();
+          // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
+          if (!imported && s_import != 0) {
+            // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
+            for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
+              uint32_t pw[kImportBatch];
+#pragma unroll
+              for (int q = 0; q < kImportBatch; ++q) {
+                const int32_t w = wb + q * kThreads + tid;
+                pw[q] = w < bw ? gload(gprev + w) : 0u;
+              }
+#pragma unroll
+              for (int q = 0; q < kImportBatch; ++q) {
+                const int32_t w = wb + q * kThreads + tid;
+                uint32_t nb = 0;
+                if (w < bw) {
+                  const uint32_t cw = bm[w];
+                  nb = pw[q] & ~cw;
+                  bm[w] = cw | pw[q];
+                  const int32_t b0 = w * 32;
+                  if (b0 + 31 <= src) nb = 0;
+                  else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
+                }
+                u_new += __popc(nb);
+                if (want_list) {
+                  const uint32_t cnt = __popc(nb);
+                  // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
+                  const uint32_t incl = wave_incl_scan_u32(cnt);
+                  const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                  uint32_t base = 0;
+                  if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
+                  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+                  uint32_t pos = base + incl - cnt;
+                  while (nb) {
+                    const int b = __ffs(nb) - 1;
+                    nb &= nb - 1;
+                    gstore(ustage + pos++, w * 32 + b);
+                  }
+                }
+              }
+            }
+            imported = true;
+            __syncthreads();
+          }
+        }
+
+        // ---- this row's totals ----
+        {
+          const uint32_t a_w = wave_sum_u32(w_all), a_wo = wave_sum_u32(w_ovl), a_x = wave_sum_u32(x_e);
+          const uint32_t a_r = wave_sum_u32(r_all), a_ro = wave_sum_u32(r_ovl);
+          const uint32_t a_us = wave_sum_u32(u_seed), a_un = wave_sum_u32(u_new);
+          if (lane == 0) {
+            if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
+            if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
+            if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
+            if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
+            if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
+            if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
+            if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
+          }
+        }
+        __syncthreads();
+        const uint64_t deg0 = (uint64_t)s_deg0;
+        uint64_t wr, rr;
+        uint32_t U;
+        if (imported) {
+          wr = wr_prev + deg_prev + (s_wall - s_wovl);
+          rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
+          U = s_unew;
+        } else if (chained) {
+          wr = wr_prev + deg_prev + s_wall;
+          rr = r_prev + 1 + (uint64_t)s_rall;
+          U = u_prev - (uint32_t)s_src_was_set + s_unew;
+        } else {
+          wr = s_wall;
+          rr = s_rall;
+          U = s_unew;
+        }
+        // the next row needs the saved bitmap if it is escalated and does not hold this one
+        const bool save = src + 1 < r1 && s_esc[li + 1] != 0 && s_hold[li + 1] == 0;
+        const bool list_ok = want_list && s_lp[li + 1] - s_lp[li] == (int64_t)U && s_ust == U;
+        if (tid == 0) {
+          if (mb != last_mat) {
+            flush_totals(mats, last_mat, tot);
+            last_mat = mb;
+          }
+          M.cnt[dir][src] = (int32_t)U;
+          tot.fill += U - s_useed;
+          tot.heavy += 1;
+          tot.x_edges += s_xe;
+          tot.x_verts += (uint64_t)s_rall + 1;
+          tot.x_out += U;
+          if (dir == 0) {
+            tot.w += deg0 + wr;
+            tot.r += rr;
+            tot.nnz_u += U;
+          } else {
+            tot.nnz_l += U;
+            // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
+            uint32_t win[4];
+            for (int q = 0; q < 4; ++q) {
+              const int32_t lo = src + 1 + 32 * q;  // first row of this window word
+              uint32_t word = 0;
+              if (lo < chunk_end) {
+                const int32_t wi = lo >> 5, sh = lo & 31;
+                const uint32_t a = bm[wi];
+                const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
+                word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
+                const int32_t valid = chunk_end - lo;  // rows of the chunk left
+                if (valid < 32) word &= (1u << valid) - 1u;
+              }
+              win[q] = word;
+            }
+            reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
+          }
+          if (want_list && !list_ok) atomicOr(&M.acc[kAccOverflow], 1ull);
+        }
+        if (list_ok) {
+          int32_t* const dst = M.pool[dir] + s_lp[li];
+          uint32_t i = tid;
+          for (; i + 3 * kThreads < U; i += 4 * kThreads) {
+            const int32_t a = gload(ustage + i), b = gload(ustage + i + kThreads);
+            const int32_t c = gload(ustage + i + 2 * kThreads), d = gload(ustage + i + 3 * kThreads);
+            gstore(dst + i, a);
+            gstore(dst + i + kThreads, b);
+            gstore(dst + i + 2 * kThreads, c);
+            gstore(dst + i + 3 * kThreads, d);
+          }
+          for (; i < U; i += kThreads) gstore(dst + i, gload(ustage + i));
+        }
+        if (save)
+          for (int32_t w = tid; w < bw; w += kThreads) gstore(gprev + w, bm[w]);
+        prev_saved = save;
+        wr_prev = wr;
+        deg_prev = deg0;
+        r_prev = rr;
+        u_prev = U;
+        prev = src;
+        __syncthreads();
       }
-      const bool save = s_save_next != 0;
-      if (save)
-        for (int32_t w = tid; w < bw; w += kThreads) gstore(gprev + w, bm[w]);
-      prev_saved = save;
-      wr_prev = wr;
-      deg_prev = deg0;
-      r_prev = rr;
-      u_prev = U;
-      prev = src;
-      __syncthreads();
+      __syncthreads();  // the next window's facts overwrite these
     }
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
