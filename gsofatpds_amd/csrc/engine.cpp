@@ -183,6 +183,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   for (int b = 0; b < nmats_; ++b)
     blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt_.chain_block - 1) / opt_.chain_block;
   nblk_total_ = blkq_base_h_[nmats_];
+  nb_max_ = (int32_t)(((int64_t)n_max_ + opt_.chain_block - 1) / opt_.chain_block);
   const int occ_l = light_occupancy();
   if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
   light_grid_ = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
@@ -216,6 +217,8 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
   GSOFA_TRY(alloc(blk_flag_, (size_t)(2 * nblk_total_), "block flags"));
+  GSOFA_TRY(alloc(blkq_base_d_, nmats_ + 1, "chain block base"));
+  GSOFA_TRY(alloc(bucket_, nb_max_, "block index histogram"));
   if (any_dup) {
     GSOFA_TRY(alloc(dup_u_, nrows_, "duplicate surplus (upper)"));
     GSOFA_TRY(alloc(dup_all_, nrows_, "duplicate surplus (row)"));
@@ -249,6 +252,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(upload(item_base_d_, item_base_h_.data(), item_base_h_.size() * 8, "upload"));
   GSOFA_TRY(upload(row_base_d_, row_base_h_.data(), row_base_h_.size() * 8, "upload"));
   GSOFA_TRY(upload(blk_base_d_, blk_base_h_.data(), blk_base_h_.size() * 8, "upload"));
+  GSOFA_TRY(upload(blkq_base_d_, blkq_base_h_.data(), blkq_base_h_.size() * 8, "upload"));
   if (alloc_stream_) {
     // pageable sources: make sure the copies consumed them before the vectors go away
     GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(uploads)");
@@ -309,6 +313,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * nrows * kEll * 2;          // ELL A, A^T
   b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
+  b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
   b += 4 * heavy_grid * (3 * (int64_t)nmax + ((int64_t)nmax + 31) / 32);  // spill regions
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
@@ -332,7 +337,11 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
             "transpose");
   GSOFA_HIP(launch_ell_build(nrows_, rowptr_t_, adj_t_, ell_t_, s), "ell A^T");
   if (ev) GSOFA_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
-  GSOFA_HIP(launch_light(mats_d_, nmats_, item_base_d_, nitems_, ws_, heavy_items_, light_grid_, s), "k_light");
+  GSOFA_HIP(launch_light(mats_d_, nmats_, item_base_d_, nitems_, light_grid_, s), "k_light");
+  // cursor_, block_sums_ and total_ are free again once the transpose is built
+  GSOFA_HIP(launch_heavy_order(nblk_total_, nmats_, blkq_base_d_, blk_flag_, nb_max_, bucket_, cursor_, block_sums_,
+                               total_, ws_, heavy_items_, s),
+            "workgroup-path queue");
   if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
   GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, spill_, s),
             "k_heavy");

@@ -117,6 +117,9 @@ class DeviceSet {
   uint32_t* blk_flag_ = nullptr;  // 2 * nblk_total
   int64_t nblk_total_ = 0, esc_words_ = 0;
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
+  int64_t* blkq_base_d_ = nullptr;
+  int32_t nb_max_ = 0;                // chain blocks of the largest matrix
+  int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries
   int32_t *pool_u_ = nullptr, *pool_l_ = nullptr;
   std::vector<int64_t> ptr_base_h_;                        // matrix b's offsets start at row_base + b

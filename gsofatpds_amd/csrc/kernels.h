@@ -53,7 +53,7 @@ struct MatDev {
   uint8_t* sup_flag;              // n: 1 where the supernode count increments
   unsigned long long* acc;        // kAccSlots
   uint8_t* esc[2];                // n: 1 where the wave path escalated the source
-  uint32_t* blk_flag[2];          // ceil(n / chain_block): block already queued for the workgroup path
+  uint32_t* blk_flag[2];          // ceil(n / chain_block): 1 where the block has an escalated row
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
 };
 
@@ -85,8 +85,14 @@ hipError_t launch_ell_build(int64_t nrows, const int32_t* rowptr, const int32_t*
 hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
                             const int32_t* adj, int32_t* col_cnt, int32_t* rowptr_t, int32_t* cursor,
                             int32_t* adj_t, int32_t* block_sums, int32_t* total, hipStream_t s);
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws,
-                        uint64_t* heavy_items, int grid, hipStream_t s);
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid,
+                        hipStream_t s);
+// The escalated blocks (blk_flag, 2 * nblk_total slots) into heavy_items in descending block
+// order; sets ws->heavy_count. bucket has nb_max entries, cursor nb_max + 1, block_sums
+// ceil(nb_max / 4096), total one int.
+hipError_t launch_heavy_order(int64_t nblk_total, int nmats, const int64_t* blkq_base, const uint32_t* blk_flag,
+                              int32_t nb_max, int32_t* bucket, int32_t* cursor, int32_t* block_sums, int32_t* total,
+                              WorkState* ws, uint64_t* heavy_items, hipStream_t s);
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap);
 hipError_t heavy_set_lds_limit(int threads, size_t bytes);
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,

@@ -341,8 +341,7 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_
 // X_s), and once such a chain escalates, its later rows escalate without an attempt.
 
 __global__ __launch_bounds__(kLightWaves * 64) void k_light(
-    const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems,
-    WorkState* __restrict__ ws, uint64_t* __restrict__ heavy_items) {
+    const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems) {
   __shared__ int32_t s_hash[kLightWaves][kHash];
   __shared__ int32_t s_q[kLightWaves][kLightQ];
   const int wave = threadIdx.x >> 6;
@@ -573,11 +572,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
       if (esc) {
         if (lane == 0) {
           gstore(M.esc[dir] + src, (uint8_t)1);
-          const int32_t hb = src / M.chain_block;
-          if (atomicOr(&M.blk_flag[dir][hb], 1u) == 0u) {
-            const unsigned long long idx = atomicAdd(&ws->heavy_count, 1ull);
-            heavy_items[idx] = encode_item((uint32_t)mb, (uint32_t)dir, (uint32_t)hb);
-          }
+          gstore(M.blk_flag[dir] + src / M.chain_block, 1u);  // the block goes to the workgroup path
         }
         carry_ok = false;
         prev_esc = true;
@@ -589,6 +584,42 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
     }
   }
   flush(last_mat);
+}
+
+// ------------------------------------------------------------------------------------------
+// The workgroup-path queue, in a fixed order: flagged blocks by descending block index (all
+// matrices of a set share chain_block, and the highest rows reach the most vertices, so the
+// costliest items start first and the tail is short), both directions and all matrices of
+// one block index together. A counting sort over block indices: histogram, scan, scatter.
+// Slot g < nblk_total is (direction 0, global block g); the rest are direction 1.
+// ------------------------------------------------------------------------------------------
+__global__ void k_heavy_hist(int64_t nblk_total, int nmats, const int64_t* __restrict__ blkq_base,
+                             const uint32_t* __restrict__ blk_flag, int32_t nb_max, int32_t* __restrict__ bucket,
+                             WorkState* __restrict__ ws) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < 2 * nblk_total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    if (blk_flag[g] == 0) continue;
+    const int64_t gb = g < nblk_total ? g : g - nblk_total;
+    const int m = find_segment(blkq_base, nmats, gb);
+    const int32_t b = (int32_t)(gb - blkq_base[m]);
+    atomicAdd(&bucket[nb_max - 1 - b], 1);
+    atomicAdd(&ws->heavy_count, 1ull);
+  }
+}
+
+__global__ void k_heavy_scatter(int64_t nblk_total, int nmats, const int64_t* __restrict__ blkq_base,
+                                const uint32_t* __restrict__ blk_flag, int32_t nb_max, int32_t* __restrict__ cursor,
+                                uint64_t* __restrict__ heavy_items) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < 2 * nblk_total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    if (blk_flag[g] == 0) continue;
+    const uint32_t dir = g < nblk_total ? 0u : 1u;
+    const int64_t gb = g < nblk_total ? g : g - nblk_total;
+    const int m = find_segment(blkq_base, nmats, gb);
+    const int32_t b = (int32_t)(gb - blkq_base[m]);
+    const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
+    heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1179,9 +1210,27 @@ hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, c
   return hipGetLastError();
 }
 
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws,
-                        uint64_t* heavy_items, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_light, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems, ws, heavy_items);
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_light, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
+  return hipGetLastError();
+}
+
+hipError_t launch_heavy_order(int64_t nblk_total, int nmats, const int64_t* blkq_base, const uint32_t* blk_flag,
+                              int32_t nb_max, int32_t* bucket, int32_t* cursor, int32_t* block_sums, int32_t* total,
+                              WorkState* ws, uint64_t* heavy_items, hipStream_t s) {
+  const int64_t nslots = 2 * nblk_total;
+  const int g = grid_for(nslots, 256, 8192);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nb_max, 256, 8192)), dim3(256), 0, s, bucket, (int64_t)nb_max, 0);
+  hipLaunchKernelGGL(k_heavy_hist, dim3(g), dim3(256), 0, s, nblk_total, nmats, blkq_base, blk_flag, nb_max, bucket, ws);
+  const int64_t per = (int64_t)kScanBlock * kScanItems;
+  const int64_t nb = (nb_max + per - 1) / per;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, block_sums);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, s, block_sums, nb, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, block_sums,
+                     cursor, nullptr, total);
+  hipLaunchKernelGGL(k_heavy_scatter, dim3(g), dim3(256), 0, s, nblk_total, nmats, blkq_base, blk_flag, nb_max, cursor,
+                     heavy_items);
   return hipGetLastError();
 }
 
