@@ -26,6 +26,7 @@ Options to_options(const gsofa_options* o) {
   r.light_wgs_per_cu = o->light_wgs_per_cu;
   r.qcap = o->queue_capacity;
   r.chain_block = o->chain_block > 0 ? o->chain_block : 0;  // 0: chosen from n and the grid
+  r.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   return r;
 }
 
@@ -181,6 +182,7 @@ void gsofa_options_default(gsofa_options* o) {
   std::memset(o, 0, sizeof(*o));
   o->chunk_size = 128;
   o->n_processes = 1;
+  o->pair_directions = -1;
 }
 
 int gsofa_plan_create(const gsofa_csr* a, const gsofa_options* opt, gsofa_plan** out) {

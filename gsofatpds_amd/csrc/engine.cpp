@@ -18,6 +18,7 @@ int prepare(const gsofa_csr* a, Prepared& out) {
   out.adj.clear();
   out.adj.reserve((size_t)a->nnz);
   out.dup_u.clear();
+  out.hold_prev = 0;
   std::vector<int32_t> row;
   bool any_dup = false;
   std::vector<int32_t> dup((size_t)n, 0), dup_all((size_t)n, 0);
@@ -32,6 +33,7 @@ int prepare(const gsofa_csr* a, Prepared& out) {
     int32_t upper_distinct = 0;
     for (int32_t c : row) upper_distinct += c > v;
     dup[v] = upper_all - upper_distinct;
+    if (v > 0 && std::binary_search(row.begin(), row.end(), v - 1)) ++out.hold_prev;
     any_dup |= dup_all[v] != 0;
     out.adj.insert(out.adj.end(), row.begin(), row.end());
     out.rowptr[v + 1] = (int32_t)out.adj.size();
@@ -41,6 +43,13 @@ int prepare(const gsofa_csr* a, Prepared& out) {
     out.dup_all.swap(dup_all);
   }
   return GSOFA_OK;
+}
+
+// Paired items pay a bitmap save per row and win where chaining by "row s holds s-1" is rare
+// (unstructured patterns); structured ones (stencils in nested-dissection order) chain directly.
+bool pair_directions(const Prepared& P, int32_t mode) {
+  if (mode >= 0) return mode > 0;
+  return P.n > 1 && 4 * P.hold_prev < (int64_t)P.n - 1;
 }
 
 int device_info(int device, DeviceInfo& info) {
@@ -141,6 +150,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)P.n + kLightBlock - 1) / kLightBlock);
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
+    paired_any_ |= pair_directions(P, opt.pair_directions) ? 1 : 0;
   }
   nrows_ = row_base_h_[nmats_];
   nnz_ = nnz_base_h_[nmats_];
@@ -156,7 +166,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : (heavy_threads_ == 1024 ? 1 : 2);
   const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
   const size_t bitmap_bytes = (size_t)bw_max_ * 4;
-  const size_t reserve = 2048;  // static LDS of the kernel (1.2 KB) plus margin
+  const size_t reserve = 3072;  // static LDS of the kernel (2.2 KB) plus margin
   if (bitmap_bytes + reserve + 256 * 8 > info.lds_per_block)
     return fail(GSOFA_ERR_CAPACITY, "n = " + std::to_string(n_max_) + " exceeds the LDS bitmap capacity of the device");
   if (opt.qcap) {
@@ -175,6 +185,8 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   heavy_grid_ = cus * occ;
   // Rows per workgroup-path item: long blocks chain more, short blocks keep every workgroup
   // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
+  // (Measured on the unstructured batch members: paired items, which carry both directions,
+  // do best with the same rule; one block per workgroup left the tail to the costliest ones.)
   if (opt_.chain_block == 0) {
     int64_t k = (int64_t)n_max_ / std::max<int64_t>(1, 2 * (int64_t)heavy_grid_);
     opt_.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(64, k));
@@ -210,9 +222,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
   GSOFA_TRY(alloc(ws_, 1, "work state"));
   GSOFA_TRY(alloc(heavy_items_, 2 * nblk_total_, "escalation list"));
-  // per resident workgroup: two frontier spill queues, the U list, the saved bitmap of the
-  // previous row (import)
-  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (3 * (size_t)n_max_ + ((size_t)n_max_ + 31) / 32),
+  // per resident workgroup: two frontier spill queues, the U list and the saved bitmap of the
+  // last row (with paired items: a U list per direction, a bitmap per direction and row parity)
+  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * ((paired_any_ ? 4 : 3) * (size_t)n_max_ +
+                                                 (paired_any_ ? 4 : 1) * (((size_t)n_max_ + 31) / 32)),
                   "heavy spill regions"));
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
@@ -284,6 +297,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.blk_flag[0] = blk_flag_ + blkq_base_h_[b];
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
     M.chain_block = opt_.chain_block;
+    M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");
@@ -297,13 +311,14 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   if (rc != GSOFA_OK) return rc;
   int64_t nrows = 0, nnz = 0, nblk = 0;
   int32_t nmax = 0;
-  bool dup = false;
+  bool dup = false, paired = false;
   for (const Prepared* P : mats) {
     nrows += P->n;
     nnz += (int64_t)P->adj.size();
     nmax = std::max(nmax, P->n);
     nblk += ((int64_t)P->n + 3) / 4;  // smallest automatic chain block
     dup |= !P->dup_u.empty();
+    paired |= pair_directions(*P, opt.pair_directions);
   }
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
   const int64_t heavy_grid = (int64_t)cus * 2;  // upper bound on resident workgroups per CU
@@ -314,7 +329,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
   b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
-  b += 4 * heavy_grid * (3 * (int64_t)nmax + ((int64_t)nmax + 31) / 32);  // spill regions
+  b += 4 * heavy_grid * ((paired ? 4 : 3) * (int64_t)nmax + (paired ? 4 : 1) * (((int64_t)nmax + 31) / 32));  // spill
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
   b += 1 << 20;                       // tables, accumulators, scan scratch
@@ -339,11 +354,12 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   if (ev) GSOFA_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
   GSOFA_HIP(launch_light(mats_d_, nmats_, item_base_d_, nitems_, light_grid_, s), "k_light");
   // cursor_, block_sums_ and total_ are free again once the transpose is built
-  GSOFA_HIP(launch_heavy_order(nblk_total_, nmats_, blkq_base_d_, blk_flag_, nb_max_, bucket_, cursor_, block_sums_,
+  GSOFA_HIP(launch_heavy_order(mats_d_, nblk_total_, nmats_, blkq_base_d_, blk_flag_, nb_max_, bucket_, cursor_, block_sums_,
                                total_, ws_, heavy_items_, s),
             "workgroup-path queue");
   if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
-  GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, spill_, s),
+  GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, paired_any_,
+                         spill_, s),
             "k_heavy");
   if (ev) GSOFA_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
   GSOFA_HIP(launch_supernodes(mats_d_, nmats_, blk_base_d_, nblocks_sup_, s), "k_supernodes");

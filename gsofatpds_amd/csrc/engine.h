@@ -21,7 +21,11 @@ struct Prepared {
   std::vector<int32_t> adj;
   std::vector<int32_t> dup_u;   // empty when the input had no duplicate upper entries
   std::vector<int32_t> dup_all; // input row length minus de-duplicated length; empty if none
+  int64_t hold_prev = 0;        // rows s >= 1 whose row holds s-1
 };
+
+// Whether the workgroup path runs both directions of a block together (Options::pair_directions).
+bool pair_directions(const Prepared& P, int32_t mode);
 
 int prepare(const gsofa_csr* a, Prepared& out);
 
@@ -35,6 +39,7 @@ struct Options {
   int32_t light_wgs_per_cu = 0; // 0 = tuned
   int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
   int32_t chain_block = 0;      // rows per workgroup-path item; 1 disables chaining; 0 = auto
+  int32_t pair_directions = -1; // 1 on, 0 off, -1 where few rows hold their predecessor
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
 };
@@ -118,6 +123,7 @@ class DeviceSet {
   int64_t nblk_total_ = 0, esc_words_ = 0;
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
   int64_t* blkq_base_d_ = nullptr;
+  int32_t paired_any_ = 0;            // some matrix runs paired workgroup-path items
   int32_t nb_max_ = 0;                // chain blocks of the largest matrix
   int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries

@@ -55,6 +55,7 @@ struct MatDev {
   uint8_t* esc[2];                // n: 1 where the wave path escalated the source
   uint32_t* blk_flag[2];          // ceil(n / chain_block): 1 where the block has an escalated row
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
+  int32_t paired;                 // 1: workgroup-path items run both directions of a block
 };
 
 // Scheduler state of one launch set (plain matrix or batch).
@@ -90,13 +91,15 @@ hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base,
 // The escalated blocks (blk_flag, 2 * nblk_total slots) into heavy_items in descending block
 // order; sets ws->heavy_count. bucket has nb_max entries, cursor nb_max + 1, block_sums
 // ceil(nb_max / 4096), total one int.
-hipError_t launch_heavy_order(int64_t nblk_total, int nmats, const int64_t* blkq_base, const uint32_t* blk_flag,
-                              int32_t nb_max, int32_t* bucket, int32_t* cursor, int32_t* block_sums, int32_t* total,
-                              WorkState* ws, uint64_t* heavy_items, hipStream_t s);
+hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats, const int64_t* blkq_base,
+                              const uint32_t* blk_flag, int32_t nb_max, int32_t* bucket, int32_t* cursor,
+                              int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s);
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap);
 hipError_t heavy_set_lds_limit(int threads, size_t bytes);
+// spill: per resident workgroup (3 n_max + ceil(n_max / 32)) ints, or (4 n_max + 4 ceil(n_max / 32))
+// when paired_any (some matrix of the set runs paired items).
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
-                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s);
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
 constexpr int kLightBlock = 8;  // consecutive rows per wave-path item

@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace gsofa {
@@ -591,32 +593,43 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
 // matrices of a set share chain_block, and the highest rows reach the most vertices, so the
 // costliest items start first and the tail is short), both directions and all matrices of
 // one block index together. A counting sort over block indices: histogram, scan, scatter.
-// Slot g < nblk_total is (direction 0, global block g); the rest are direction 1.
+// Slot g < nblk_total is (direction 0, global block g); the rest are direction 1. A paired
+// matrix gets one item per block (both directions), from its direction-0 slot.
+__device__ __forceinline__ bool heavy_slot(const MatDev* mats, int64_t nblk_total, int nmats,
+                                           const int64_t* blkq_base, const uint32_t* blk_flag, int64_t g,
+                                           int& m, uint32_t& dir, int32_t& b) {
+  const int64_t gb = g < nblk_total ? g : g - nblk_total;
+  m = find_segment(blkq_base, nmats, gb);
+  b = (int32_t)(gb - blkq_base[m]);
+  dir = g < nblk_total ? 0u : 1u;
+  if (mats[m].paired) return dir == 0 && (blk_flag[gb] | blk_flag[gb + nblk_total]) != 0;
+  return blk_flag[g] != 0;
+}
+
 // ------------------------------------------------------------------------------------------
-__global__ void k_heavy_hist(int64_t nblk_total, int nmats, const int64_t* __restrict__ blkq_base,
-                             const uint32_t* __restrict__ blk_flag, int32_t nb_max, int32_t* __restrict__ bucket,
-                             WorkState* __restrict__ ws) {
+__global__ void k_heavy_hist(const MatDev* __restrict__ mats, int64_t nblk_total, int nmats,
+                             const int64_t* __restrict__ blkq_base, const uint32_t* __restrict__ blk_flag,
+                             int32_t nb_max, int32_t* __restrict__ bucket, WorkState* __restrict__ ws) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < 2 * nblk_total;
        g += (int64_t)gridDim.x * blockDim.x) {
-    if (blk_flag[g] == 0) continue;
-    const int64_t gb = g < nblk_total ? g : g - nblk_total;
-    const int m = find_segment(blkq_base, nmats, gb);
-    const int32_t b = (int32_t)(gb - blkq_base[m]);
+    int m;
+    uint32_t dir;
+    int32_t b;
+    if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
     atomicAdd(&bucket[nb_max - 1 - b], 1);
     atomicAdd(&ws->heavy_count, 1ull);
   }
 }
 
-__global__ void k_heavy_scatter(int64_t nblk_total, int nmats, const int64_t* __restrict__ blkq_base,
-                                const uint32_t* __restrict__ blk_flag, int32_t nb_max, int32_t* __restrict__ cursor,
-                                uint64_t* __restrict__ heavy_items) {
+__global__ void k_heavy_scatter(const MatDev* __restrict__ mats, int64_t nblk_total, int nmats,
+                                const int64_t* __restrict__ blkq_base, const uint32_t* __restrict__ blk_flag,
+                                int32_t nb_max, int32_t* __restrict__ cursor, uint64_t* __restrict__ heavy_items) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < 2 * nblk_total;
        g += (int64_t)gridDim.x * blockDim.x) {
-    if (blk_flag[g] == 0) continue;
-    const uint32_t dir = g < nblk_total ? 0u : 1u;
-    const int64_t gb = g < nblk_total ? g : g - nblk_total;
-    const int m = find_segment(blkq_base, nmats, gb);
-    const int32_t b = (int32_t)(gb - blkq_base[m]);
+    int m;
+    uint32_t dir;
+    int32_t b;
+    if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
     const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
     heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b);
   }
@@ -635,9 +648,9 @@ __device__ __forceinline__ void qput(int32_t* l, int32_t* g, int32_t qcap, int32
   else gstore(g + (e - qcap), v);
 }
 
-struct Totals {
-  uint64_t fill = 0, w = 0, r = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
-  uint64_t x_edges = 0, x_verts = 0, x_out = 0;
+struct Totals {  // cleared with Totals{}
+  uint64_t fill, w, r, nnz_u, nnz_l, heavy;
+  uint64_t x_edges, x_verts, x_out;
 };
 // Every k_heavy total is also the workgroup path's own share: W, R and F entries found there.
 
@@ -659,26 +672,35 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
   t = Totals{};
 }
 
-// One workgroup per item = a block of chain_block consecutive rows of one direction; it
-// traverses the block's escalated rows in increasing order and reuses the previous row's
-// reached set X_{s-1} = {s-1} + R_{s-1} + U_{s-1} whenever s-1 is in R_s, because then
-// R_{s-1}, s-1 and all their neighbours belong to X_s:
-//   * direct chaining: row s holds s-1. The bitmap carries over unchanged and only vertices
-//     new to s are expanded.
-//   * import: otherwise, if the previous row's bitmap was saved, s is traversed from scratch
-//     and, the moment s-1 is reached, the saved X_{s-1} is OR-ed in at the next level
-//     boundary; queued vertices already in X_{s-1} are then dropped (their neighbours are in).
+// One workgroup per item = a block of chain_block consecutive rows; it traverses the block's
+// escalated rows in increasing order and reuses the previous row's reached set
+// X_{s-1} = {s-1} + R_{s-1} + U_{s-1} whenever s-1 is in R_s, because then R_{s-1}, s-1 and
+// all their neighbours belong to X_s:
+//   * chaining: row s holds s-1 (or s-1 is known to be in R_s, below). The bitmap starts as
+//     X_{s-1} (carried in LDS, or copied back from the saved copy) and only vertices new to s
+//     are expanded.
+//   * import: otherwise, if X_{s-1} was saved, s is traversed from scratch and, the moment s-1
+//     is reached, X_{s-1} is OR-ed in at the next level boundary; queued vertices already in
+//     X_{s-1} are then dropped (their neighbours are in).
 // W_R(s) = sum of deg over R_s and |R_s| are carried exactly: the previous row's values, plus
 // s-1, plus the vertices expanded here that are not in X_{s-1}.
+//
+// Paired items (M.paired) run both directions of the block, row by row (s dir 0, s dir 1,
+// s+1 dir 0, ...), and save every row's bitmap. Then s-1 in R_s is often known before any
+// expansion: if an entry u < s-1 of row s (in this direction) lies in the other direction's
+// R'_{s-1}, u reaches s-1 through vertices < s-1, so s -> u -> ... -> s-1 stays below s.
+// That turns the expensive late imports of unstructured patterns (where row s rarely holds
+// s-1) into chaining. Plain items run one direction and save a bitmap only for an import.
 //
 // The kernel is bound by dependent global-memory round trips, one workgroup per CU: the
 // per-row facts (escalated? holds its predecessor? structure offsets) are read for a window
 // of up to kHeavyWin rows in one round into LDS, loads of one expansion step are issued
-// together, and the import reads the saved bitmap kImportBatch words per thread per round.
+// together, and bitmap copies move kImportBatch words per thread per round.
 constexpr int kHeavyWin = 64;
 constexpr int kImportBatch = 8;
 
-template <int kThreads>
+// kPairedAny: some matrix of the set runs paired items (otherwise that code is compiled out).
+template <int kThreads, bool kPairedAny>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_heavy(const MatDev* __restrict__ mats,
                                                     WorkState* __restrict__ ws,
                                                     const uint64_t* __restrict__ heavy_items,
@@ -693,11 +715,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ uint32_t s_ust, s_unew, s_useed;
   __shared__ unsigned long long s_wall, s_wovl, s_xe;
   __shared__ uint32_t s_rall, s_rovl;
-  __shared__ int32_t s_deg0, s_import, s_src_was_set;
+  __shared__ int32_t s_deg0, s_import, s_src_was_set, s_early;
   __shared__ uint32_t s_scan[kWaves];
-  __shared__ int32_t s_esc[kHeavyWin + 1];   // row w0 + i was escalated
-  __shared__ int32_t s_hold[kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
-  __shared__ int64_t s_lp[kHeavyWin + 2];    // structure offsets of rows w0 .. w0 + np
+  __shared__ int32_t s_esc[2][kHeavyWin + 1];   // row w0 + i was escalated (per direction)
+  __shared__ int32_t s_hold[2][kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
+  __shared__ int64_t s_lp[2][kHeavyWin + 2];    // structure offsets of rows w0 .. w0 + np
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
@@ -705,15 +727,25 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   uint32_t* bm = dyn;
   int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
   int32_t* const ql1 = ql0 + qcap;
-  const int64_t region_len = 3 * n_max + (n_max + 31) / 32;
+  // Global region of this workgroup: two frontier spill queues, the U list of the last row of
+  // each direction, saved bitmaps: one, or with paired items four (direction x row parity, so
+  // that X_{s-1} of direction 0 survives while row s of direction 0 is saved).
+  const int64_t nbw = (n_max + 31) / 32;
+  const int64_t region_len = (kPairedAny ? 4 : 3) * n_max + (kPairedAny ? 4 : 1) * nbw;
   int32_t* const region = spill_all + (int64_t)blockIdx.x * region_len;
   int32_t* const qg0 = region;
   int32_t* const qg1 = region + n_max;
-  int32_t* const ustage = region + 2 * n_max;                                  // U list of the current row
-  uint32_t* const gprev = reinterpret_cast<uint32_t*>(region + 3 * n_max);    // saved X_{s-1}
+  int32_t* const ustage0 = region + 2 * n_max;
+  int32_t* const ustage1 = kPairedAny ? region + 3 * n_max : ustage0;
+  uint32_t* const gsave = reinterpret_cast<uint32_t*>(region + (kPairedAny ? 4 : 3) * n_max);
+  // the saved bitmap of (row r, direction d)
+  auto gslot = [&](int dd, int32_t r) -> uint32_t* {
+    return kPairedAny ? gsave + (int64_t)(2 * dd + (r & 1)) * nbw : gsave;
+  };
 
   int last_mat = -1;
-  Totals tot;  // thread 0 only
+  __shared__ Totals tot;  // thread 0 only (in LDS: keeps the registers for the traversal)
+  if (tid == 0) tot = Totals{};
 
   const unsigned long long total = ws->heavy_count;
   while (true) {
@@ -723,397 +755,478 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     if (idx >= total) break;
     const uint64_t item = heavy_items[idx];
     const int mb = (int)(item >> 33);
-    const int dir = (int)((item >> 32) & 1);
+    const int item_dir = (int)((item >> 32) & 1);
     const int32_t blk = (int32_t)(item & 0xffffffffu);
     const MatDev& M = mats[mb];
+    const bool paired = kPairedAny && M.paired != 0;
+    const int ndir = paired ? 2 : 1;
     const int32_t n = M.n;
     const int32_t bw = (n + 31) >> 5;
     const int32_t r0 = blk * M.chain_block;
     const int32_t r1 = min(n, r0 + M.chain_block);
-    const bool want_list = M.list_ptr[dir] != nullptr;
+    const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
-    int32_t prev = -1;                     // previous row traversed in this block
-    bool prev_saved = false;               // gprev holds X_prev
-    uint64_t wr_prev = 0, deg_prev = 0;    // W_R and input degree of prev
-    uint64_t r_prev = 0;                   // |R_prev|
-    uint32_t u_prev = 0;                   // |U_prev|
+    // per direction (index = direction; a plain item uses its own direction's slot): the last
+    // row traversed in this block, whether its bitmap is saved, W_R, input degree, |R|, |U|,
+    // length of its staged U list; with paired items also the row before (pprev) and its save
+    int32_t prev0 = -1, prev1 = -1, pprev0 = -1, pprev1 = -1;
+    bool saved0 = false, saved1 = false, psaved0 = false, psaved1 = false;
+    uint64_t wrp0 = 0, wrp1 = 0, degp0 = 0, degp1 = 0, rp0 = 0, rp1 = 0;
+    uint32_t up0 = 0, up1 = 0, ul0 = 0, ul1 = 0;
+    int32_t lds_row = -1, lds_dir = -1;  // what the LDS bitmap holds
     for (int32_t w0 = r0; w0 < r1; w0 += kHeavyWin) {
       const int32_t we = min(r1, w0 + kHeavyWin);
       const int32_t np = min(r1, we + 1) - w0;  // rows read: the window plus one look-ahead row
-      // ---- window facts, one round of loads ----
-      int32_t esc_v = 0;
-      int64_t lp_v = 0;
-      if (tid < np) esc_v = gload(M.esc[dir] + w0 + tid);
-      if (want_list && tid <= np) lp_v = gload(M.list_ptr[dir] + w0 + tid);
-      int32_t ev[kPfRounds];
+      // ---- window facts, one round of loads per direction ----
+      for (int di = 0; di < ndir; ++di) {
+        const int dd = paired ? di : item_dir;
+        int32_t esc_v = 0;
+        int64_t lp_v = 0;
+        if (tid < np) esc_v = gload(M.esc[dd] + w0 + tid);
+        if (want_list && tid <= np) lp_v = gload(M.list_ptr[dd] + w0 + tid);
+        int32_t ev[kPfRounds];
 #pragma unroll
-      for (int q = 0; q < kPfRounds; ++q) {
-        const int32_t i = q * kThreads + tid;
-        ev[q] = i < np * kEll ? gload(M.ell[dir] + (int64_t)w0 * kEll + i) : kPad;
-      }
-#pragma unroll
-      for (int q = 0; q < kPfRounds; ++q) {
-        const int32_t i = q * kThreads + tid;
-        const int32_t row = w0 + (i >> 4);
-        bool hit = row > 0 && ev[q] == row - 1;
-        if ((i & (kEll - 1)) == kEll - 1 && ev[q] <= -2 && i < np * kEll) {  // long row: its CSR tail
-          for (int32_t p = -(ev[q] + 2), pe = gload(M.rowptr[dir] + row + 1); p < pe; ++p)
-            hit |= gload(M.adj[dir] + p) == row - 1;
+        for (int q = 0; q < kPfRounds; ++q) {
+          const int32_t i = q * kThreads + tid;
+          ev[q] = i < np * kEll ? gload(M.ell[dd] + (int64_t)w0 * kEll + i) : kPad;
         }
-        const uint64_t b = __ballot(hit);
-        if ((i & (kEll - 1)) == 0 && i < np * kEll) s_hold[i >> 4] = ((b >> (lane & 48)) & 0xFFFFull) != 0;
+#pragma unroll
+        for (int q = 0; q < kPfRounds; ++q) {
+          const int32_t i = q * kThreads + tid;
+          const int32_t row = w0 + (i >> 4);
+          bool hit = row > 0 && ev[q] == row - 1;
+          if ((i & (kEll - 1)) == kEll - 1 && ev[q] <= -2 && i < np * kEll) {  // long row: its CSR tail
+            for (int32_t p = -(ev[q] + 2), pe = gload(M.rowptr[dd] + row + 1); p < pe; ++p)
+              hit |= gload(M.adj[dd] + p) == row - 1;
+          }
+          const uint64_t b = __ballot(hit);
+          if ((i & (kEll - 1)) == 0 && i < np * kEll) s_hold[dd][i >> 4] = ((b >> (lane & 48)) & 0xFFFFull) != 0;
+        }
+        if (tid < np) s_esc[dd][tid] = esc_v;
+        if (want_list && tid <= np) s_lp[dd][tid] = lp_v;
       }
-      if (tid < np) s_esc[tid] = esc_v;
-      if (want_list && tid <= np) s_lp[tid] = lp_v;
       __syncthreads();
 
       for (int32_t src = w0; src < we; ++src) {
         const int32_t li = src - w0;
-        if (s_esc[li] == 0) continue;  // the wave path finished this row
-        const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
-        const bool follows = prev >= 0 && prev == src - 1;
-        const bool chained = follows && s_hold[li] != 0;
-        const bool has_prev = !chained && follows && prev_saved;
+        for (int di = 0; di < ndir; ++di) {
+          const int d = paired ? di : item_dir;
+          if (s_esc[d][li] == 0) continue;  // the wave path finished this row
+          const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
+          int32_t* const ustage = d ? ustage1 : ustage0;
+          uint32_t* const gprev = gslot(d, src - 1);  // X_{src-1} of this direction, when saved
+          const int32_t prev_d = d ? prev1 : prev0;
+          const bool saved_d = d ? saved1 : saved0;
+          const bool follows = prev_d >= 0 && prev_d == src - 1;
+          const bool hold = s_hold[d][li] != 0;
+          const bool lds_prev = lds_row == src - 1 && lds_dir == d;  // LDS still holds X_{src-1}
 
-        // ---- start: bitmap, staged U list, counters (one barrier) ----
-        if (!chained) {
-          const int32_t sw = src >> 5;
-          const uint32_t sb = 1u << (src & 31);
-          for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
-          if (tid == 0) {
-            s_ust = 0;
-            s_src_was_set = 0;
-          }
-        } else {
-          if (want_list) {
-            // keep the carried U entries above src, in place (reads of a round finish before its writes)
-            const uint32_t len = s_ust;
-            uint32_t kept = 0;
-            for (uint32_t base = 0; base < len; base += kThreads) {
-              const uint32_t i = base + tid;
-              const int32_t m = i < len ? gload(ustage + i) : -1;
-              const bool keep = m > src;
-              const uint64_t bal = __ballot(keep);
-              if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
-              __syncthreads();
-              uint32_t before = 0, all = 0;
-              for (int w = 0; w < kWaves; ++w) {
-                if (w < wave) before += s_scan[w];
-                all += s_scan[w];
+          const bool can_chain = follows && (lds_prev || saved_d);  // X_{src-1} is at hand
+          // ---- paired: is s-1 in R_s by the other direction's R'_{s-1}? ----
+          bool early = false;
+          if (paired && can_chain && !hold) {
+            const int o = 1 - d;
+            // X'_{src-1} of the other direction: its last row, or (when that is src) the one before
+            const bool other_ok = ((o ? prev1 : prev0) == src - 1 && (o ? saved1 : saved0)) ||
+                                  ((o ? pprev1 : pprev0) == src - 1 && (o ? psaved1 : psaved0));
+            if (other_ok) {
+              const uint32_t* const gother = gslot(o, src - 1);
+              if (wave == 0) {
+                const int32_t e = lane < kEll ? gload(M.ell[d] + (int64_t)src * kEll + lane) : kPad;
+                bool hit = e >= 0 && e < src - 1 && ((gload(gother + (e >> 5)) >> (e & 31)) & 1u) != 0;
+                if (lane == kEll - 1 && e <= -2) {
+                  for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !hit; ++p) {
+                    const int32_t u = gload(M.adj[d] + p);
+                    hit = u < src - 1 && ((gload(gother + (u >> 5)) >> (u & 31)) & 1u) != 0;
+                  }
+                }
+                const uint64_t b = __ballot(hit);
+                if (lane == 0) s_early = b != 0;
               }
-              if (keep) gstore(ustage + kept + before + rank_below(bal), m);
-              kept += all;
               __syncthreads();
+              early = s_early != 0;
             }
-            if (tid == 0) s_ust = kept;
           }
-          if (tid == 0) {
+          const bool chained = can_chain && (hold || early);
+          const bool chain_lds = chained && lds_prev;
+          const bool chain_copy = chained && !lds_prev;
+          const bool has_prev = !chained && follows && saved_d;
+
+          // ---- start: bitmap, staged U list, counters ----
+          if (!chained) {
+            const int32_t sw = src >> 5;
             const uint32_t sb = 1u << (src & 31);
-            const uint32_t old = bm[src >> 5];
-            s_src_was_set = (old & sb) != 0;
-            bm[src >> 5] = old | sb;
-          }
-        }
-        if (tid == 0) {
-          ql0[0] = src;
-          s_cnt[0] = 1;
-          s_cnt[1] = 0;
-          s_cnt[2] = 0;
-          s_unew = 0;
-          s_useed = 0;
-          s_wall = s_wovl = s_xe = 0;
-          s_rall = s_rovl = 0;
-          s_deg0 = 0;
-          s_import = 0;
-        }
-        __syncthreads();
-
-        // this row's sums (each vertex is expanded at most once per row, so every sum stays
-        // below the input nnz < 2^31)
-        uint32_t w_all = 0, w_ovl = 0, x_e = 0;
-        uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
-        // Whether X_{src-1} was imported: a register, identical in every thread (each thread
-        // takes the same branches below). A shared flag written inside the import block could be
-        // seen by a slower wave before it evaluated the block's condition.
-        bool imported = false;
-        for (int level = 0;; ++level) {
-          const int32_t ncur = s_cnt[level % 3];
-          if (ncur == 0) break;
-          if (tid == 0) s_cnt[(level + 2) % 3] = 0;
-          const bool odd = (level & 1) != 0;
-          const int32_t* const cur_l = odd ? ql1 : ql0;
-          const int32_t* const cur_g = odd ? qg1 : qg0;
-          int32_t* const nxt_l = odd ? ql0 : ql1;
-          int32_t* const nxt_g = odd ? qg0 : qg1;
-          int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
-          const bool seed = level == 0;
-          for (int32_t base = 0; base < ncur; base += kThreads) {
-            const int32_t e = base + tid;
-            int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
-            // issued together: membership in the saved X_{s-1}, the row, its duplicate surplus
-            const bool chk = v >= 0 && !seed && has_prev;
-            const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
-            RowCursor rc;
-            load_row(M, dir, v, rc);
-            const uint32_t dupv = (v >= 0 && dir == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
-            const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
-            if (in_prev && imported) {  // covered by the imported X_{s-1}
-              v = -1;
-#pragma unroll
-              for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
-              rc.ov_pos = rc.ov_end = 0;
+            for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
+            if (tid == 0) {
+              s_ust = 0;
+              s_src_was_set = 0;
             }
-            uint32_t deg = 0;
-            // Claim the bits of 8 slots at a time: the LDS atomics are independent and in flight
-            // together (measured faster than a plain read first). Rows are packed, so the second
-            // group is skipped when no lane has a ninth entry. New entries below src go to the
-            // next level, new entries above src are U.
-            uint32_t mask_r = 0, mask_u = 0;
+          } else {
+            if (chain_copy) {
+              // X_{src-1} back from its saved copy, with the bit of src set
+              for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
+                uint32_t pw[kImportBatch];
 #pragma unroll
-            for (int h = 0; h < kEll; h += 8) {
-              if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
-              uint32_t old[8];
+                for (int q = 0; q < kImportBatch; ++q) {
+                  const int32_t w = wb + q * kThreads + tid;
+                  pw[q] = w < bw ? gload(gprev + w) : 0u;
+                }
 #pragma unroll
-              for (int j = 0; j < 8; ++j) {
-                const int32_t m = rc.e[h + j];
-                old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
-              }
-#pragma unroll
-              for (int j = 0; j < 8; ++j) {
-                const int32_t m = rc.e[h + j];
-                if (m >= 0) {
-                  ++deg;
-                  if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
-                  if (((old[j] >> (m & 31)) & 1u) == 0) {
-                    if (has_prev && m == src - 1) s_import = 1;  // import X_{src-1} instead of expanding it
-                    else if (m < src) mask_r |= 1u << (h + j);
-                    else if (m > src) mask_u |= 1u << (h + j);
+                for (int q = 0; q < kImportBatch; ++q) {
+                  const int32_t w = wb + q * kThreads + tid;
+                  if (w < bw) {
+                    uint32_t x = pw[q];
+                    if (w == (src >> 5)) {
+                      s_src_was_set = (x >> (src & 31)) & 1u;
+                      x |= 1u << (src & 31);
+                    }
+                    bm[w] = x;
                   }
                 }
               }
             }
-            // One queue reservation per wave (exclusive scan of the per-lane counts).
-            {
-              const uint32_t c = __popc(mask_r);
-              const uint32_t incl = wave_incl_scan_u32(c);
-              const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-              if (tot_w != 0) {
-                int32_t basepos = 0;
-                if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
-                int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
-#pragma unroll
-                for (int j = 0; j < kEll; ++j)
-                  if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
-              }
-            }
-            u_new += __popc(mask_u);
             if (want_list) {
-              const uint32_t c = __popc(mask_u);
-              const uint32_t incl = wave_incl_scan_u32(c);
-              const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-              if (tot_w != 0) {
-                uint32_t basepos = 0;
-                if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
-                uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
-#pragma unroll
-                for (int j = 0; j < kEll; ++j)
-                  if ((mask_u >> j) & 1u) gstore(ustage + pos++, rc.e[j]);
+              // keep the carried U entries above src, in place (reads of a round finish before its writes)
+              const uint32_t len = d ? ul1 : ul0;
+              uint32_t kept = 0;
+              for (uint32_t base = 0; base < len; base += kThreads) {
+                const uint32_t i = base + tid;
+                const int32_t m = i < len ? gload(ustage + i) : -1;
+                const bool keep = m > src;
+                const uint64_t bal = __ballot(keep);
+                if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
+                __syncthreads();
+                uint32_t before = 0, all = 0;
+                for (int w = 0; w < kWaves; ++w) {
+                  if (w < wave) before += s_scan[w];
+                  all += s_scan[w];
+                }
+                if (keep) gstore(ustage + kept + before + rank_below(bal), m);
+                kept += all;
+                __syncthreads();
               }
+              if (tid == 0) s_ust = kept;
             }
-            // the CSR tail of a long row, one entry per lane per step
-            auto visit = [&](int32_t m) {
-              const bool active = m >= 0;
-              deg += active;
-              if (seed && m > src) ++u_seed;
-              bool isnew = false;
-              if (active) {
-                const uint32_t bit = 1u << (m & 31);
-                uint32_t* wp = &bm[m >> 5];
-                if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+            if (chain_lds && tid == 0) {
+              const uint32_t sb = 1u << (src & 31);
+              const uint32_t old = bm[src >> 5];
+              s_src_was_set = (old & sb) != 0;
+              bm[src >> 5] = old | sb;
+            }
+          }
+          if (tid == 0) {
+            ql0[0] = src;
+            s_cnt[0] = 1;
+            s_cnt[1] = 0;
+            s_cnt[2] = 0;
+            s_unew = 0;
+            s_useed = 0;
+            s_wall = s_wovl = s_xe = 0;
+            s_rall = s_rovl = 0;
+            s_deg0 = 0;
+            s_import = 0;
+          }
+          __syncthreads();
+
+          // this row's sums (each vertex is expanded at most once per row, so every sum stays
+          // below the input nnz < 2^31)
+          uint32_t w_all = 0, w_ovl = 0, x_e = 0;
+          uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
+          // Whether X_{src-1} was imported: a register, identical in every thread (each thread
+          // takes the same branches below). A shared flag written inside the import block could be
+          // seen by a slower wave before it evaluated the block's condition.
+          bool imported = false;
+          for (int level = 0;; ++level) {
+            const int32_t ncur = s_cnt[level % 3];
+            if (ncur == 0) break;
+            if (tid == 0) s_cnt[(level + 2) % 3] = 0;
+            const bool odd = (level & 1) != 0;
+            const int32_t* const cur_l = odd ? ql1 : ql0;
+            const int32_t* const cur_g = odd ? qg1 : qg0;
+            int32_t* const nxt_l = odd ? ql0 : ql1;
+            int32_t* const nxt_g = odd ? qg0 : qg1;
+            int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
+            const bool seed = level == 0;
+            for (int32_t base = 0; base < ncur; base += kThreads) {
+              const int32_t e = base + tid;
+              int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+              // issued together: membership in the saved X_{s-1}, the row, its duplicate surplus
+              const bool chk = v >= 0 && !seed && has_prev;
+              const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
+              RowCursor rc;
+              load_row(M, d, v, rc);
+              const uint32_t dupv = (v >= 0 && d == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
+              const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
+              if (in_prev && imported) {  // covered by the imported X_{s-1}
+                v = -1;
+#pragma unroll
+                for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
+                rc.ov_pos = rc.ov_end = 0;
               }
-              if (isnew && has_prev && m == src - 1) {
-                s_import = 1;
-                isnew = false;
-              }
-              const bool is_r = isnew && m < src;
-              const bool is_u = isnew && m > src;
-              const uint64_t mr = __ballot(is_r);
-              if (mr) {
-                const int leader = __ffsll((unsigned long long)mr) - 1;
-                int32_t basepos = 0;
-                if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
-                basepos = __shfl(basepos, leader, 64);
-                if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
-              }
-              u_new += is_u;
-              if (want_list) {
-                const uint64_t mu = __ballot(is_u);
-                if (mu) {
-                  const int leader = __ffsll((unsigned long long)mu) - 1;
-                  uint32_t basepos = 0;
-                  if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
-                  basepos = __shfl(basepos, leader, 64);
-                  if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+              uint32_t deg = 0;
+              // Claim the bits of 8 slots at a time: the LDS atomics are independent and in flight
+              // together (measured faster than a plain read first). Rows are packed, so the second
+              // group is skipped when no lane has a ninth entry. New entries below src go to the
+              // next level, new entries above src are U.
+              uint32_t mask_r = 0, mask_u = 0;
+#pragma unroll
+              for (int h = 0; h < kEll; h += 8) {
+                if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
+                uint32_t old[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                  const int32_t m = rc.e[h + j];
+                  old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                  const int32_t m = rc.e[h + j];
+                  if (m >= 0) {
+                    ++deg;
+                    if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
+                    if (((old[j] >> (m & 31)) & 1u) == 0) {
+                      if (has_prev && m == src - 1) s_import = 1;  // import X_{src-1} instead of expanding it
+                      else if (m < src) mask_r |= 1u << (h + j);
+                      else if (m > src) mask_u |= 1u << (h + j);
+                    }
+                  }
                 }
               }
-            };
-            while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
-              int32_t m = kPad;
-              if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
-              visit(m);
-            }
-            x_e += deg;
-            if (v >= 0) deg += dupv;
-            if (seed) {
-              if (tid == 0) s_deg0 = (int32_t)deg;
-            } else if (v >= 0) {
-              w_all += deg;
-              r_all += 1;
-              if (in_prev) {
-                w_ovl += deg;
-                r_ovl += 1;
+              // One queue reservation per wave (exclusive scan of the per-lane counts).
+              {
+                const uint32_t c = __popc(mask_r);
+                const uint32_t incl = wave_incl_scan_u32(c);
+                const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                if (tot_w != 0) {
+                  int32_t basepos = 0;
+                  if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
+                  int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
+#pragma unroll
+                  for (int j = 0; j < kEll; ++j)
+                    if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
+                }
               }
+              u_new += __popc(mask_u);
+              if (want_list) {
+                const uint32_t c = __popc(mask_u);
+                const uint32_t incl = wave_incl_scan_u32(c);
+                const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                if (tot_w != 0) {
+                  uint32_t basepos = 0;
+                  if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
+                  uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
+#pragma unroll
+                  for (int j = 0; j < kEll; ++j)
+                    if ((mask_u >> j) & 1u) gstore(ustage + pos++, rc.e[j]);
+                }
+              }
+              // the CSR tail of a long row, one entry per lane per step
+              auto visit = [&](int32_t m) {
+                const bool active = m >= 0;
+                deg += active;
+                if (seed && m > src) ++u_seed;
+                bool isnew = false;
+                if (active) {
+                  const uint32_t bit = 1u << (m & 31);
+                  uint32_t* wp = &bm[m >> 5];
+                  if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+                }
+                if (isnew && has_prev && m == src - 1) {
+                  s_import = 1;
+                  isnew = false;
+                }
+                const bool is_r = isnew && m < src;
+                const bool is_u = isnew && m > src;
+                const uint64_t mr = __ballot(is_r);
+                if (mr) {
+                  const int leader = __ffsll((unsigned long long)mr) - 1;
+                  int32_t basepos = 0;
+                  if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
+                  basepos = __shfl(basepos, leader, 64);
+                  if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
+                }
+                u_new += is_u;
+                if (want_list) {
+                  const uint64_t mu = __ballot(is_u);
+                  if (mu) {
+                    const int leader = __ffsll((unsigned long long)mu) - 1;
+                    uint32_t basepos = 0;
+                    if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
+                    basepos = __shfl(basepos, leader, 64);
+                    if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+                  }
+                }
+              };
+              while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+                int32_t m = kPad;
+                if (rc.ov_pos < rc.ov_end) m = gload(M.adj[d] + rc.ov_pos++);
+                visit(m);
+              }
+              x_e += deg;
+              if (v >= 0) deg += dupv;
+              if (seed) {
+                if (tid == 0) s_deg0 = (int32_t)deg;
+              } else if (v >= 0) {
+                w_all += deg;
+                r_all += 1;
+                if (in_prev) {
+                  w_ovl += deg;
+                  r_ovl += 1;
+                }
+              }
+            }
+            __syncthreads();
+            // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
+            if (!imported && s_import != 0) {
+              // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
+              for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
+                uint32_t pw[kImportBatch];
+#pragma unroll
+                for (int q = 0; q < kImportBatch; ++q) {
+                  const int32_t w = wb + q * kThreads + tid;
+                  pw[q] = w < bw ? gload(gprev + w) : 0u;
+                }
+#pragma unroll
+                for (int q = 0; q < kImportBatch; ++q) {
+                  const int32_t w = wb + q * kThreads + tid;
+                  uint32_t nb = 0;
+                  if (w < bw) {
+                    const uint32_t cw = bm[w];
+                    nb = pw[q] & ~cw;
+                    bm[w] = cw | pw[q];
+                    const int32_t b0 = w * 32;
+                    if (b0 + 31 <= src) nb = 0;
+                    else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
+                  }
+                  u_new += __popc(nb);
+                  if (want_list) {
+                    const uint32_t cnt = __popc(nb);
+                    // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
+                    const uint32_t incl = wave_incl_scan_u32(cnt);
+                    const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                    uint32_t base = 0;
+                    if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
+                    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+                    uint32_t pos = base + incl - cnt;
+                    while (nb) {
+                      const int b = __ffs(nb) - 1;
+                      nb &= nb - 1;
+                      gstore(ustage + pos++, w * 32 + b);
+                    }
+                  }
+                }
+              }
+              imported = true;
+              __syncthreads();
+            }
+          }
+
+          // ---- this row's totals ----
+          {
+            const uint32_t a_w = wave_sum_u32(w_all), a_wo = wave_sum_u32(w_ovl), a_x = wave_sum_u32(x_e);
+            const uint32_t a_r = wave_sum_u32(r_all), a_ro = wave_sum_u32(r_ovl);
+            const uint32_t a_us = wave_sum_u32(u_seed), a_un = wave_sum_u32(u_new);
+            if (lane == 0) {
+              if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
+              if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
+              if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
+              if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
+              if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
+              if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
+              if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
             }
           }
           __syncthreads();
-          // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
-          if (!imported && s_import != 0) {
-            // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
-            for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
-              uint32_t pw[kImportBatch];
-#pragma unroll
-              for (int q = 0; q < kImportBatch; ++q) {
-                const int32_t w = wb + q * kThreads + tid;
-                pw[q] = w < bw ? gload(gprev + w) : 0u;
-              }
-#pragma unroll
-              for (int q = 0; q < kImportBatch; ++q) {
-                const int32_t w = wb + q * kThreads + tid;
-                uint32_t nb = 0;
-                if (w < bw) {
-                  const uint32_t cw = bm[w];
-                  nb = pw[q] & ~cw;
-                  bm[w] = cw | pw[q];
-                  const int32_t b0 = w * 32;
-                  if (b0 + 31 <= src) nb = 0;
-                  else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
-                }
-                u_new += __popc(nb);
-                if (want_list) {
-                  const uint32_t cnt = __popc(nb);
-                  // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
-                  const uint32_t incl = wave_incl_scan_u32(cnt);
-                  const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-                  uint32_t base = 0;
-                  if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
-                  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
-                  uint32_t pos = base + incl - cnt;
-                  while (nb) {
-                    const int b = __ffs(nb) - 1;
-                    nb &= nb - 1;
-                    gstore(ustage + pos++, w * 32 + b);
-                  }
-                }
-              }
-            }
-            imported = true;
-            __syncthreads();
-          }
-        }
-
-        // ---- this row's totals ----
-        {
-          const uint32_t a_w = wave_sum_u32(w_all), a_wo = wave_sum_u32(w_ovl), a_x = wave_sum_u32(x_e);
-          const uint32_t a_r = wave_sum_u32(r_all), a_ro = wave_sum_u32(r_ovl);
-          const uint32_t a_us = wave_sum_u32(u_seed), a_un = wave_sum_u32(u_new);
-          if (lane == 0) {
-            if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
-            if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
-            if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
-            if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
-            if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
-            if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
-            if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
-          }
-        }
-        __syncthreads();
-        const uint64_t deg0 = (uint64_t)s_deg0;
-        uint64_t wr, rr;
-        uint32_t U;
-        if (imported) {
-          wr = wr_prev + deg_prev + (s_wall - s_wovl);
-          rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
-          U = s_unew;
-        } else if (chained) {
-          wr = wr_prev + deg_prev + s_wall;
-          rr = r_prev + 1 + (uint64_t)s_rall;
-          U = u_prev - (uint32_t)s_src_was_set + s_unew;
-        } else {
-          wr = s_wall;
-          rr = s_rall;
-          U = s_unew;
-        }
-        // the next row needs the saved bitmap if it is escalated and does not hold this one
-        const bool save = src + 1 < r1 && s_esc[li + 1] != 0 && s_hold[li + 1] == 0;
-        const bool list_ok = want_list && s_lp[li + 1] - s_lp[li] == (int64_t)U && s_ust == U;
-        if (tid == 0) {
-          if (mb != last_mat) {
-            flush_totals(mats, last_mat, tot);
-            last_mat = mb;
-          }
-          M.cnt[dir][src] = (int32_t)U;
-          tot.fill += U - s_useed;
-          tot.heavy += 1;
-          tot.x_edges += s_xe;
-          tot.x_verts += (uint64_t)s_rall + 1;
-          tot.x_out += U;
-          if (dir == 0) {
-            tot.w += deg0 + wr;
-            tot.r += rr;
-            tot.nnz_u += U;
+          const uint64_t deg0 = (uint64_t)s_deg0;
+          const uint64_t wr_prev = d ? wrp1 : wrp0, deg_prev = d ? degp1 : degp0, r_prev = d ? rp1 : rp0;
+          const uint32_t u_prev = d ? up1 : up0;
+          uint64_t wr, rr;
+          uint32_t U;
+          if (imported) {
+            wr = wr_prev + deg_prev + (s_wall - s_wovl);
+            rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
+            U = s_unew;
+          } else if (chained) {
+            wr = wr_prev + deg_prev + s_wall;
+            rr = r_prev + 1 + (uint64_t)s_rall;
+            U = u_prev - (uint32_t)s_src_was_set + s_unew;
           } else {
-            tot.nnz_l += U;
-            // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
-            uint32_t win[4];
-            for (int q = 0; q < 4; ++q) {
-              const int32_t lo = src + 1 + 32 * q;  // first row of this window word
-              uint32_t word = 0;
-              if (lo < chunk_end) {
-                const int32_t wi = lo >> 5, sh = lo & 31;
-                const uint32_t a = bm[wi];
-                const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
-                word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
-                const int32_t valid = chunk_end - lo;  // rows of the chunk left
-                if (valid < 32) word &= (1u << valid) - 1u;
-              }
-              win[q] = word;
+            wr = s_wall;
+            rr = s_rall;
+            U = s_unew;
+          }
+          // Save the bitmap for the next row: paired, whenever the next row is escalated in
+          // either direction (the LDS copy will not survive the other direction); plain, when
+          // the next row is escalated and does not hold this one (only an import can use it).
+          const bool next_in = src + 1 < r1;
+          const bool save = paired ? next_in && (s_esc[0][li + 1] != 0 || s_esc[1][li + 1] != 0)
+                                   : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+          const uint32_t ust_now = s_ust;
+          const bool list_ok = want_list && s_lp[d][li + 1] - s_lp[d][li] == (int64_t)U && ust_now == U;
+          if (tid == 0) {
+            if (mb != last_mat) {
+              flush_totals(mats, last_mat, tot);
+              last_mat = mb;
             }
-            reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
+            M.cnt[d][src] = (int32_t)U;
+            tot.fill += U - s_useed;
+            tot.heavy += 1;
+            tot.x_edges += s_xe;
+            tot.x_verts += (uint64_t)s_rall + 1;
+            tot.x_out += U;
+            if (d == 0) {
+              tot.w += deg0 + wr;
+              tot.r += rr;
+              tot.nnz_u += U;
+            } else {
+              tot.nnz_l += U;
+              // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
+              uint32_t win[4];
+              for (int q = 0; q < 4; ++q) {
+                const int32_t lo = src + 1 + 32 * q;  // first row of this window word
+                uint32_t word = 0;
+                if (lo < chunk_end) {
+                  const int32_t wi = lo >> 5, sh = lo & 31;
+                  const uint32_t a = bm[wi];
+                  const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
+                  word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
+                  const int32_t valid = chunk_end - lo;  // rows of the chunk left
+                  if (valid < 32) word &= (1u << valid) - 1u;
+                }
+                win[q] = word;
+              }
+              reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
+            }
+            if (want_list && !list_ok) atomicOr(&M.acc[kAccOverflow], 1ull);
           }
-          if (want_list && !list_ok) atomicOr(&M.acc[kAccOverflow], 1ull);
-        }
-        if (list_ok) {
-          int32_t* const dst = M.pool[dir] + s_lp[li];
-          uint32_t i = tid;
-          for (; i + 3 * kThreads < U; i += 4 * kThreads) {
-            const int32_t a = gload(ustage + i), b = gload(ustage + i + kThreads);
-            const int32_t c = gload(ustage + i + 2 * kThreads), d = gload(ustage + i + 3 * kThreads);
-            gstore(dst + i, a);
-            gstore(dst + i + kThreads, b);
-            gstore(dst + i + 2 * kThreads, c);
-            gstore(dst + i + 3 * kThreads, d);
+          if (list_ok) {
+            int32_t* const dst = M.pool[d] + s_lp[d][li];
+            uint32_t i = tid;
+            for (; i + 3 * kThreads < U; i += 4 * kThreads) {
+              const int32_t a = gload(ustage + i), b = gload(ustage + i + kThreads);
+              const int32_t c = gload(ustage + i + 2 * kThreads), e = gload(ustage + i + 3 * kThreads);
+              gstore(dst + i, a);
+              gstore(dst + i + kThreads, b);
+              gstore(dst + i + 2 * kThreads, c);
+              gstore(dst + i + 3 * kThreads, e);
+            }
+            for (; i < U; i += kThreads) gstore(dst + i, gload(ustage + i));
           }
-          for (; i < U; i += kThreads) gstore(dst + i, gload(ustage + i));
+          if (save) {
+            uint32_t* const gout = gslot(d, src);
+            for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm[w]);
+          }
+          if (d) {
+            pprev1 = prev1; psaved1 = saved1;
+            prev1 = src; saved1 = save; wrp1 = wr; degp1 = deg0; rp1 = rr; up1 = U; ul1 = ust_now;
+          } else {
+            pprev0 = prev0; psaved0 = saved0;
+            prev0 = src; saved0 = save; wrp0 = wr; degp0 = deg0; rp0 = rr; up0 = U; ul0 = ust_now;
+          }
+          lds_row = src;
+          lds_dir = d;
+          __syncthreads();
         }
-        if (save)
-          for (int32_t w = tid; w < bw; w += kThreads) gstore(gprev + w, bm[w]);
-        prev_saved = save;
-        wr_prev = wr;
-        deg_prev = deg0;
-        r_prev = rr;
-        u_prev = U;
-        prev = src;
-        __syncthreads();
       }
       __syncthreads();  // the next window's facts overwrite these
     }
@@ -1216,52 +1329,70 @@ hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base,
   return hipGetLastError();
 }
 
-hipError_t launch_heavy_order(int64_t nblk_total, int nmats, const int64_t* blkq_base, const uint32_t* blk_flag,
-                              int32_t nb_max, int32_t* bucket, int32_t* cursor, int32_t* block_sums, int32_t* total,
-                              WorkState* ws, uint64_t* heavy_items, hipStream_t s) {
+hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats, const int64_t* blkq_base,
+                              const uint32_t* blk_flag, int32_t nb_max, int32_t* bucket, int32_t* cursor,
+                              int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s) {
   const int64_t nslots = 2 * nblk_total;
   const int g = grid_for(nslots, 256, 8192);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nb_max, 256, 8192)), dim3(256), 0, s, bucket, (int64_t)nb_max, 0);
-  hipLaunchKernelGGL(k_heavy_hist, dim3(g), dim3(256), 0, s, nblk_total, nmats, blkq_base, blk_flag, nb_max, bucket, ws);
+  hipLaunchKernelGGL(k_heavy_hist, dim3(g), dim3(256), 0, s, mats, nblk_total, nmats, blkq_base, blk_flag, nb_max, bucket,
+                     ws);
   const int64_t per = (int64_t)kScanBlock * kScanItems;
   const int64_t nb = (nb_max + per - 1) / per;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, block_sums);
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, s, block_sums, nb, total);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, block_sums,
                      cursor, nullptr, total);
-  hipLaunchKernelGGL(k_heavy_scatter, dim3(g), dim3(256), 0, s, nblk_total, nmats, blkq_base, blk_flag, nb_max, cursor,
-                     heavy_items);
+  hipLaunchKernelGGL(k_heavy_scatter, dim3(g), dim3(256), 0, s, mats, nblk_total, nmats, blkq_base, blk_flag, nb_max,
+                     cursor, heavy_items);
   return hipGetLastError();
 }
 
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap) { return (size_t)bw_max * 4 + (size_t)qcap * 8; }
 
-hipError_t heavy_set_lds_limit(int threads, size_t bytes) {
+template <bool kP>
+hipError_t heavy_set_lds_limit_t(int threads, size_t bytes) {
+  const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
   switch (threads) {
-    case 1024: return hipFuncSetAttribute((const void*)k_heavy<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    case 512: return hipFuncSetAttribute((const void*)k_heavy<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    case 256: return hipFuncSetAttribute((const void*)k_heavy<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    case 1024: return hipFuncSetAttribute((const void*)k_heavy<1024, kP>, a, (int)bytes);
+    case 512: return hipFuncSetAttribute((const void*)k_heavy<512, kP>, a, (int)bytes);
+    case 256: return hipFuncSetAttribute((const void*)k_heavy<256, kP>, a, (int)bytes);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
-                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s) {
+hipError_t heavy_set_lds_limit(int threads, size_t bytes) {
+  const hipError_t e = heavy_set_lds_limit_t<false>(threads, bytes);
+  return e != hipSuccess ? e : heavy_set_lds_limit_t<true>(threads, bytes);
+}
+
+template <bool kP>
+hipError_t launch_heavy_t(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
+                          int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s) {
   const size_t lds = heavy_lds_bytes(bw_max, qcap);
   switch (threads) {
     case 1024:
-      hipLaunchKernelGGL(k_heavy<1024>, dim3(grid), dim3(1024), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      hipLaunchKernelGGL((k_heavy<1024, kP>), dim3(grid), dim3(1024), lds, s, mats, ws, heavy_items, bw_max, qcap,
+                         n_max, spill);
       break;
     case 512:
-      hipLaunchKernelGGL(k_heavy<512>, dim3(grid), dim3(512), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      hipLaunchKernelGGL((k_heavy<512, kP>), dim3(grid), dim3(512), lds, s, mats, ws, heavy_items, bw_max, qcap,
+                         n_max, spill);
       break;
     case 256:
-      hipLaunchKernelGGL(k_heavy<256>, dim3(grid), dim3(256), lds, s, mats, ws, heavy_items, bw_max, qcap, n_max, spill);
+      hipLaunchKernelGGL((k_heavy<256, kP>), dim3(grid), dim3(256), lds, s, mats, ws, heavy_items, bw_max, qcap,
+                         n_max, spill);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s) {
+  if (paired_any) return launch_heavy_t<true>(threads, grid, mats, ws, heavy_items, bw_max, qcap, n_max, spill, s);
+  return launch_heavy_t<false>(threads, grid, mats, ws, heavy_items, bw_max, qcap, n_max, spill, s);
 }
 
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
@@ -1285,16 +1416,22 @@ hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipS
   return hipGetLastError();
 }
 
-int heavy_occupancy(int threads, size_t lds_bytes) {
+template <bool kP>
+int heavy_occupancy_t(int threads, size_t lds_bytes) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (threads) {
-    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<1024>, 1024, lds_bytes); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<512>, 512, lds_bytes); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<256>, 256, lds_bytes); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<1024, kP>, 1024, lds_bytes); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<512, kP>, 512, lds_bytes); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<256, kP>, 256, lds_bytes); break;
     default: break;
   }
   return e == hipSuccess ? nb : 0;
+}
+
+// Resident workgroups per CU of both variants (the grid must fit either).
+int heavy_occupancy(int threads, size_t lds_bytes) {
+  return std::min(heavy_occupancy_t<false>(threads, lds_bytes), heavy_occupancy_t<true>(threads, lds_bytes));
 }
 
 int light_occupancy() {

@@ -259,6 +259,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   r->opt.heavy_threads = o->heavy_threads;
   r->opt.qcap = o->queue_capacity;
   r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 0;
+  r->opt.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   r->opt.structure = 0;
   DeviceInfo info;
   int rc = device_info(srv->device, info);

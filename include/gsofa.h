@@ -119,6 +119,9 @@ typedef struct gsofa_options {
   int32_t light_wgs_per_cu; /* tuning: resident hash-path workgroups per CU, 0 = automatic      */
   int32_t queue_capacity;   /* tuning: LDS frontier entries per level buffer, 0 = automatic     */
   int32_t chain_block;      /* tuning: rows per workgroup-path item (1 = no chaining), 0 = auto */
+  int32_t pair_directions;  /* workgroup-path items run both directions of a block and learn
+                               s-1 in R_s from the other direction: 1 on, 0 off, -1 (default)
+                               on where fewer than a quarter of the rows hold their predecessor */
 } gsofa_options;
 
 typedef struct gsofa_result {

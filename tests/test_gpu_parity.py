@@ -135,12 +135,43 @@ def test_chaining_blocks_grid3d_small(chain_block):
     check(p, S=128, label=f"grid3d 14^3 chain {chain_block}", chain_block=chain_block)
 
 
+@pytest.mark.parametrize("chain_block", [1, 3, 64])
+def test_paired_directions_forced(chain_block):
+    """Workgroup-path items running both directions of a block (the other direction's reach set
+    decides s-1 in R_s before any expansion), forced on structured and random patterns."""
+    check(g.generate_workload(g.WL_GRID2D_4096), label=f"grid2d_4096 paired chain {chain_block}",
+          pair_directions=1, chain_block=chain_block)
+    rng = np.random.default_rng(21)
+    n = 3000
+    D = rng.random((n, n)) < 0.002
+    np.fill_diagonal(D, rng.random(n) < 0.9)
+    D[rng.integers(0, n, 10)] |= rng.random((10, n)) < 0.03  # long rows (CSR tails)
+    D[:, 2500] |= rng.random(n) < 0.05                        # a long column
+    check(g.Pattern.from_dense(D), label=f"random long rows paired chain {chain_block}",
+          pair_directions=1, chain_block=chain_block)
+
+
+@pytest.mark.parametrize("pair", [0, 1])
+def test_paired_directions_reference_dataset(pair):
+    check(load_ds(), S=128, label=f"ds_11948 pair {pair}", pair_directions=pair)
+
+
+@pytest.mark.parametrize("member", [7, 11, 1])
+def test_erdos_renyi_members_pair_automatically(member):
+    """Unstructured members (rows rarely hold their predecessor) take paired items by default."""
+    p = g.generate_workload(g.WL_BATCH256, member)
+    check(p, structure=p.n < 3000, label=f"batch member {member} (auto)")
+    if p.n < 3000:
+        check(p, structure=True, label=f"batch member {member} unpaired", pair_directions=0)
+
+
 def test_tuning_configurations_are_identical():
     p = load_ds()
     base = None
     for opt in (dict(), dict(heavy_threads=256), dict(heavy_threads=1024), dict(queue_capacity=256),
                 dict(light_wgs_per_cu=1, heavy_wgs_per_cu=1), dict(chain_block=1), dict(chain_block=7),
-                dict(chain_block=1024, queue_capacity=256)):
+                dict(chain_block=1024, queue_capacity=256), dict(pair_directions=1),
+                dict(pair_directions=1, chain_block=3, heavy_threads=256)):
         say(f"[tuning] {opt}")
         with g.Plan(p, **opt) as plan:
             plan.run()
