@@ -34,9 +34,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_heavy_grid2d_659033.json"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the dominant kernel (optional)")
-    ap.add_argument("--batch-members", type=int, default=0,
+    ap.add_argument("--batch-members", type=int, default=256,
                     help="members of the 256-matrix batch to run batched vs plain (0 = skip)")
-    ap.add_argument("--served-requests", type=int, default=0,
+    ap.add_argument("--served-requests", type=int, default=32,
                     help="requests of the Poisson stream to serve (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=4)
