@@ -86,6 +86,8 @@ _SIGNATURES = {
     "gsofa_host_matrix_free": (None, [C.c_void_p]),
     "gsofa_read_ascii": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "gsofa_write_ascii": (C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p, C.c_char_p]),
+    "gsofa_read_binary": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gsofa_write_binary": (C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p, C.c_int32, C.c_int32]),
     "gsofa_symbfact_min_id": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
     "gsofa_device_query": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -228,6 +230,18 @@ def read_ascii(beg_file: str, end_file: str, csr_file: str) -> Pattern:
 def write_ascii(p: Pattern, beg_file: str, end_file: str, csr_file: str) -> None:
     c = p.csr()
     _check(lib().gsofa_write_ascii(C.byref(c), beg_file.encode(), end_file.encode(), csr_file.encode()))
+
+
+def read_binary(beg_file: str, csr_file: str, index_bytes: int = 8, vert_bytes: int = 8) -> Pattern:
+    """The reference's binary CSR pair (row offsets, column ids) at the given integer widths."""
+    h = C.c_void_p()
+    _check(lib().gsofa_read_binary(beg_file.encode(), csr_file.encode(), index_bytes, vert_bytes, C.byref(h)))
+    return _take_host_matrix(h)
+
+
+def write_binary(p: Pattern, beg_file: str, csr_file: str, index_bytes: int = 8, vert_bytes: int = 8) -> None:
+    c = p.csr()
+    _check(lib().gsofa_write_binary(C.byref(c), beg_file.encode(), csr_file.encode(), index_bytes, vert_bytes))
 
 
 def symbfact_min_id(argv: Sequence[str], rank: int = 0):

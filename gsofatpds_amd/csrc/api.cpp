@@ -514,6 +514,116 @@ int gsofa_write_ascii(const gsofa_csr* a, const char* beg_file, const char* end_
   return GSOFA_OK;
 }
 
+// ---- binary CSR -------------------------------------------------------------------------------
+
+static int read_file(const char* path, std::vector<unsigned char>& buf) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  buf.clear();
+  unsigned char chunk[1 << 16];
+  size_t got;
+  while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+  const bool err = std::ferror(f) != 0;
+  std::fclose(f);
+  if (err) return fail(GSOFA_ERR_IO, std::string("read error on ") + path);
+  return GSOFA_OK;
+}
+
+static int64_t word_at(const unsigned char* p, int32_t bytes) {
+  if (bytes == 4) {
+    int32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+  }
+  int64_t v;
+  std::memcpy(&v, p, 8);
+  return v;
+}
+
+static bool write_words(FILE* f, const int64_t* v, size_t count, int32_t bytes) {
+  for (size_t i = 0; i < count; ++i) {
+    if (bytes == 4) {
+      const int32_t x = (int32_t)v[i];
+      if (std::fwrite(&x, 4, 1, f) != 1) return false;
+    } else if (std::fwrite(&v[i], 8, 1, f) != 1) {
+      return false;
+    }
+  }
+  return true;
+}
+
+int gsofa_read_binary(const char* beg_file, const char* csr_file, int32_t index_bytes, int32_t vert_bytes,
+                      gsofa_host_matrix** out) {
+  if (!beg_file || !csr_file || !out) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if ((index_bytes != 4 && index_bytes != 8) || (vert_bytes != 4 && vert_bytes != 8))
+    return fail(GSOFA_ERR_INVALID_ARG, "integer widths must be 4 or 8 bytes");
+  std::vector<unsigned char> b, c;
+  int rc = read_file(beg_file, b);
+  if (rc != GSOFA_OK) return rc;
+  if (b.size() % (size_t)index_bytes != 0 || b.size() < 2 * (size_t)index_bytes)
+    return fail(GSOFA_ERR_IO, "beg file must hold n+1 offsets (n >= 1)");
+  const int64_t n = (int64_t)(b.size() / index_bytes) - 1;  // vert_count = fsize / sizeof - 1 (graph.hpp:18)
+  if (n > INT32_MAX) return fail(GSOFA_ERR_CAPACITY, "n must fit in int32");
+  std::vector<int64_t> off((size_t)n + 1);
+  for (int64_t v = 0; v <= n; ++v) off[v] = word_at(b.data() + v * index_bytes, index_bytes);
+  for (int64_t v = 0; v < n; ++v)
+    if (off[v] < 0 || off[v + 1] < off[v]) return fail(GSOFA_ERR_IO, "row offsets must be non-negative and non-decreasing");
+  const int64_t E = off[n];  // edge_count = beg_pos[vert_count] (graph.hpp:34)
+  if (E > INT32_MAX) return fail(GSOFA_ERR_CAPACITY, "edge count must fit in int32");
+  rc = read_file(csr_file, c);
+  if (rc != GSOFA_OK) return rc;
+  if ((int64_t)(c.size() / vert_bytes) < E) return fail(GSOFA_ERR_IO, "csr file holds fewer column ids than the last offset");
+  std::unique_ptr<gsofa_host_matrix> h(new (std::nothrow) gsofa_host_matrix);
+  if (!h) return fail(GSOFA_ERR_OUT_OF_MEMORY, "host matrix");
+  HostMatrix& M = h->m;
+  M.n = (int32_t)n;
+  M.adj.resize((size_t)E);
+  M.beg.resize((size_t)n);
+  M.end.resize((size_t)n);
+  for (int64_t k = 0; k < E; ++k) {
+    const int64_t x = word_at(c.data() + k * vert_bytes, vert_bytes);
+    if (x < 0 || x >= n) return fail(GSOFA_ERR_IO, "column id out of range in " + std::string(csr_file));
+    M.adj[k] = (int32_t)x;
+  }
+  for (int64_t v = 0; v < n; ++v) {
+    M.beg[v] = (int32_t)off[v];
+    M.end[v] = (int32_t)off[v + 1];
+  }
+  gsofa_csr view = M.view();
+  rc = validate_csr(&view);
+  if (rc != GSOFA_OK) return rc;
+  *out = h.release();
+  return GSOFA_OK;
+}
+
+int gsofa_write_binary(const gsofa_csr* a, const char* beg_file, const char* csr_file, int32_t index_bytes,
+                       int32_t vert_bytes) {
+  int rc = validate_csr(a);
+  if (rc != GSOFA_OK) return rc;
+  if (!beg_file || !csr_file) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if ((index_bytes != 4 && index_bytes != 8) || (vert_bytes != 4 && vert_bytes != 8))
+    return fail(GSOFA_ERR_INVALID_ARG, "integer widths must be 4 or 8 bytes");
+  // rows back to back in row order (the input may hold them anywhere in adj)
+  std::vector<int64_t> off((size_t)a->n + 1, 0), col;
+  col.reserve((size_t)a->nnz);
+  for (int32_t v = 0; v < a->n; ++v) {
+    for (int32_t k = a->beg[v]; k < a->end[v]; ++k) col.push_back(a->adj[k]);
+    off[v + 1] = (int64_t)col.size();
+  }
+  FILE* f = std::fopen(beg_file, "wb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot create ") + beg_file);
+  bool ok = write_words(f, off.data(), off.size(), index_bytes);
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(GSOFA_ERR_IO, std::string("write error on ") + beg_file);
+  f = std::fopen(csr_file, "wb");
+  if (!f) return fail(GSOFA_ERR_IO, std::string("cannot create ") + csr_file);
+  ok = write_words(f, col.data(), col.size(), vert_bytes);
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(GSOFA_ERR_IO, std::string("write error on ") + csr_file);
+  return GSOFA_OK;
+}
+
 // ---- the drop-in entry ------------------------------------------------------------------------
 
 int gsofa_symbfact_min_id(int argc, char** argv, int rank, uint64_t* fill_count, double* time_ms,

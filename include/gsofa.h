@@ -76,6 +76,16 @@ void gsofa_host_matrix_free(gsofa_host_matrix* m);
 int gsofa_read_ascii(const char* beg_file, const char* end_file, const char* csr_file, gsofa_host_matrix** out);
 int gsofa_write_ascii(const gsofa_csr* a, const char* beg_file, const char* end_file, const char* csr_file);
 
+/* Reference binary CSR (gSoFa_SOAComparison/graph.hpp:4-104, a template over its file integer
+ * types): beg file = n+1 row offsets of index_bytes each, csr file = the column ids of rows
+ * 0..n-1 back to back, vert_bytes each, native byte order; n = beg file size / index_bytes - 1,
+ * the edge count is the last offset. Widths are 4 or 8. The reference's optional weight file is
+ * not read (the symbolic path has no values). Errors return codes (the reference asserts). */
+int gsofa_read_binary(const char* beg_file, const char* csr_file, int32_t index_bytes, int32_t vert_bytes,
+                      gsofa_host_matrix** out);
+int gsofa_write_binary(const gsofa_csr* a, const char* beg_file, const char* csr_file, int32_t index_bytes,
+                       int32_t vert_bytes);
+
 /* ---- the drop-in entry ---------------------------------------------------------------------
  * Replaces void symbfact_min_id(int args, char** argv, int myrank, ull_t& fill_count,
  * double& time, int& N_supernodes, ull_t& Process_TEPS)

@@ -116,6 +116,56 @@ def test_ascii_errors(tmp_path):
     assert e.value.code == g.ERR_BAD_CSR  # column id 9 >= n = 3
 
 
+@pytest.mark.parametrize("ib,vb", [(8, 8), (4, 4), (8, 4)])
+def test_binary_round_trip(tmp_path, ib, vb):
+    """Reference binary CSR pair: n+1 offsets, then the rows back to back (rows given anywhere
+    in adj come out contiguous and in row order)."""
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    order = np.random.default_rng(3).permutation(p.n)
+    adj, beg, end, pos = [], np.zeros(p.n, np.int32), np.zeros(p.n, np.int32), 0
+    for v in order:  # non-contiguous input rows
+        r = p.adj[p.beg[v]:p.end[v]]
+        beg[v], end[v] = pos, pos + len(r)
+        adj.append(r)
+        pos += len(r)
+    q = g.Pattern(p.n, beg, end, np.concatenate(adj))
+    b, c = str(tmp_path / "beg.bin"), str(tmp_path / "csr.bin")
+    g.write_binary(q, b, c, ib, vb)
+    assert os.path.getsize(b) == (p.n + 1) * ib and os.path.getsize(c) == p.nnz * vb
+    off = np.fromfile(b, dtype=np.int64 if ib == 8 else np.int32)
+    assert off[0] == 0 and off[-1] == p.nnz and np.all(np.diff(off) == p.end - p.beg)
+    r = g.read_binary(b, c, ib, vb)
+    assert r.n == p.n and np.array_equal(r.beg, p.beg) and np.array_equal(r.end, p.end)
+    assert np.array_equal(r.adj, p.adj)
+
+
+def test_binary_errors(tmp_path):
+    b, c = tmp_path / "beg.bin", tmp_path / "csr.bin"
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c))
+    assert e.value.code == g.ERR_IO                      # missing file
+    np.array([0, 2, 4, 3], np.int64).tofile(b)           # decreasing offsets
+    np.array([0, 1, 1, 2], np.int64).tofile(c)
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c))
+    assert e.value.code == g.ERR_IO
+    np.array([0, 2, 4, 6], np.int64).tofile(b)           # csr holds 4 ids, offsets need 6
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c))
+    assert e.value.code == g.ERR_IO
+    np.array([0, 1, 1, 2, 2, 5], np.int64).tofile(c)     # id 5 >= n = 3
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c))
+    assert e.value.code == g.ERR_IO
+    b.write_bytes(b"\x00" * 13)                          # not a whole number of offsets
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c))
+    assert e.value.code == g.ERR_IO
+    with pytest.raises(g.GsofaError) as e:
+        g.read_binary(str(b), str(c), 2, 8)
+    assert e.value.code == g.ERR_INVALID_ARG
+
+
 def bad_patterns():
     ok = g.Pattern.from_dense(np.eye(4, dtype=bool))
     col = g.Pattern(4, ok.beg.copy(), ok.end.copy(), ok.adj.copy())
