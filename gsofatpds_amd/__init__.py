@@ -445,16 +445,19 @@ class Batch:
 
 class ServerOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64),
-                ("cu_share", C.c_int32)]
+                ("cu_share", C.c_int32), ("max_group", C.c_int32)]
 
 
 class Server:
     """Continuous-batching request loop: ``gsofa_server_*``."""
 
-    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 0, device: int = 0, cu_share: int = 0):
+    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 0, device: int = 0, cu_share: int = 0,
+                 max_group: int = 0):
         o = ServerOptions()
         lib().gsofa_server_options_default(C.byref(o))
         o.slots, o.hbm_budget_bytes, o.device, o.cu_share = slots, hbm_budget_bytes, device, cu_share
+        if max_group:
+            o.max_group = max_group
         h = C.c_void_p()
         _check(lib().gsofa_server_start(C.byref(o), C.byref(h)))
         self._h = h

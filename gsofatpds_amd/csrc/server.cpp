@@ -2,12 +2,14 @@
 //
 // Requests (one pattern each) are prepared on the submitting thread, queued FIFO, and admitted
 // onto one of `slots` HIP streams as soon as a slot is free and their device footprint fits the
-// HBM budget. Each admitted request gets its own device state, allocated and freed
-// stream-ordered on its slot (hipMallocAsync / hipFreeAsync: no device-wide synchronisation).
-// Requests on different slots run concurrently: their workgroups fill the CUs others leave
-// free (small requests launch few workgroups; a large one's tail frees CUs progressively), or,
-// with cu_share set, each request's persistent grids are sized for that many CUs. A completion thread polls the slots' events, reads each result on
-// its own stream and releases the slot. Latency is submit-to-result on the host clock.
+// HBM budget. A slot takes the front request together with the requests queued behind it
+// (same options, FIFO order, up to max_group of them, within the budget) as one group: one
+// device set for all of them, i.e. one launch sequence, as a batch. Device state is allocated
+// and freed stream-ordered on the slot (hipMallocAsync / hipFreeAsync: no device-wide
+// synchronisation). Groups on different slots run concurrently: their workgroups fill the CUs
+// others leave free, or, with cu_share set, each group's persistent grids are sized for that
+// many CUs. A completion thread polls the slots' events, reads the results of a finished group
+// on its stream and releases the slot. Latency is submit-to-result on the host clock.
 // No counterpart in the reference (it runs one matrix per process invocation).
 #include <hip/hip_runtime_api.h>
 
@@ -43,18 +45,27 @@ struct Request {
   std::string error_text;
   Clock::time_point t_submit, t_admit, t_done;
   MatResult result;
-  double device_ms = 0.0;
+  double device_ms = 0.0;  // of the request's group
   uint64_t nzu_digest = 0, sup_digest = 0;
-  std::unique_ptr<DeviceSet> set;
-  int slot = -1;
-  std::atomic<bool> launched{false};  // set after the slot's done event was recorded
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t start = nullptr, done = nullptr;
-  Request* req = nullptr;
+  std::vector<Request*> group;  // empty: the slot is free
+  std::unique_ptr<DeviceSet> set;
+  int64_t footprint = 0;        // charged against the budget while the group runs
+  int error = GSOFA_OK;
+  std::string error_text;
+  std::atomic<bool> launched{false};  // set after the slot's done event was recorded
 };
+
+bool same_options(const Options& a, const Options& b) {
+  return a.chunk == b.chunk && a.n_processes == b.n_processes && a.structure == b.structure &&
+         a.device == b.device && a.heavy_threads == b.heavy_threads && a.heavy_wgs_per_cu == b.heavy_wgs_per_cu &&
+         a.light_wgs_per_cu == b.light_wgs_per_cu && a.qcap == b.qcap && a.chain_block == b.chain_block &&
+         a.pair_directions == b.pair_directions && a.cu_share == b.cu_share;
+}
 
 uint64_t fnv1a(const void* data, size_t bytes) {
   const unsigned char* p = static_cast<const unsigned char*>(data);
@@ -71,9 +82,10 @@ uint64_t fnv1a(const void* data, size_t bytes) {
 struct gsofa_server {
   int device = 0;
   int cu_share = 0;
+  int max_group = 16;
   int64_t budget = 0;
   std::atomic<int64_t> in_use{0};
-  std::vector<Slot> slots;
+  std::vector<std::unique_ptr<Slot>> slots;
   std::mutex mu;
   std::condition_variable cv_admit, cv_done;
   std::deque<Request*> pending;
@@ -87,6 +99,11 @@ struct gsofa_server {
   void admit_loop();
   void done_loop();
   void finish(Slot& s);
+  Slot* free_slot() {
+    for (auto& s : slots)
+      if (s->group.empty()) return s.get();
+    return nullptr;
+  }
 };
 
 void gsofa_server::admit_loop() {
@@ -98,86 +115,115 @@ void gsofa_server::admit_loop() {
       if (pending.empty()) return false;
       Request* r = pending.front();
       if (r->cancel_requested) return true;
-      bool free_slot = false;
-      for (auto& s : slots) free_slot |= s.req == nullptr;
-      return free_slot && in_use.load() + r->footprint <= budget;
+      return free_slot() != nullptr && in_use.load() + r->footprint <= budget;
     });
     if (stopping) return;
     Request* r = pending.front();
-    pending.pop_front();
     if (r->cancel_requested) {
+      pending.pop_front();
       r->state = State::Cancelled;
       cv_done.notify_all();
       continue;
     }
-    int si = -1;
-    for (size_t k = 0; k < slots.size(); ++k)
-      if (!slots[k].req) {
-        si = (int)k;
-        break;
+    // the group: the front request and the ones queued behind it, in order, while options
+    // match and the sum of their footprints fits (the group's own footprint is at most that sum)
+    std::vector<Request*> grp{r};
+    int64_t fp = r->footprint;
+    pending.pop_front();
+    while (!pending.empty() && (int)grp.size() < max_group) {
+      Request* q = pending.front();
+      if (q->cancel_requested) {
+        pending.pop_front();
+        q->state = State::Cancelled;
+        cv_done.notify_all();
+        continue;
       }
-    Slot& s = slots[si];
-    s.req = r;
-    r->slot = si;
-    r->state = State::Running;
-    r->t_admit = Clock::now();
-    in_use += r->footprint;
-    ++running;
+      if (!same_options(q->opt, r->opt) || in_use.load() + fp + q->footprint > budget) break;
+      grp.push_back(q);
+      fp += q->footprint;
+      pending.pop_front();
+    }
+    Slot& s = *free_slot();
+    s.group = grp;
+    s.footprint = fp;
+    s.error = GSOFA_OK;
+    s.error_text.clear();
+    s.launched.store(false, std::memory_order_relaxed);
+    const Clock::time_point now = Clock::now();
+    for (Request* q : grp) {
+      q->state = State::Running;
+      q->t_admit = now;
+    }
+    in_use += fp;
+    running += (int)grp.size();
     peak_running = std::max(peak_running, running);
     lk.unlock();
     // allocation, upload and launch outside the lock; all on the slot's stream
+    std::vector<const Prepared*> mats;
+    mats.reserve(grp.size());
+    for (Request* q : grp) mats.push_back(&q->prep);
     int rc = GSOFA_OK;
-    r->set.reset(new (std::nothrow) DeviceSet);
-    if (!r->set) rc = fail(GSOFA_ERR_OUT_OF_MEMORY, "device set");
-    if (rc == GSOFA_OK) rc = r->set->create({&r->prep}, r->opt, s.stream);
+    s.set.reset(new (std::nothrow) DeviceSet);
+    if (!s.set) rc = fail(GSOFA_ERR_OUT_OF_MEMORY, "device set");
+    if (rc == GSOFA_OK) rc = s.set->create(mats, r->opt, s.stream);
     if (rc == GSOFA_OK && hipEventRecord(s.start, s.stream) != hipSuccess) rc = fail(GSOFA_ERR_HIP, "hipEventRecord");
-    if (rc == GSOFA_OK) rc = r->set->enqueue(s.stream);
+    if (rc == GSOFA_OK) rc = s.set->enqueue(s.stream);
     if (rc != GSOFA_OK) {
-      r->error = rc;
-      r->error_text = gsofa_last_error();
+      s.error = rc;
+      s.error_text = gsofa_last_error();
     }
     if (hipEventRecord(s.done, s.stream) != hipSuccess && rc == GSOFA_OK) {
-      r->error = GSOFA_ERR_HIP;
-      r->error_text = "hipEventRecord";
+      s.error = GSOFA_ERR_HIP;
+      s.error_text = "hipEventRecord";
     }
-    r->launched.store(true, std::memory_order_release);
+    s.launched.store(true, std::memory_order_release);
     lk.lock();
   }
 }
 
 void gsofa_server::finish(Slot& s) {
-  Request* r = s.req;
-  if (r->error == GSOFA_OK) {
-    float ms = 0.f;
+  std::vector<MatResult> res;
+  float ms = 0.f;
+  if (s.error == GSOFA_OK) {
     (void)hipEventElapsedTime(&ms, s.start, s.done);
-    r->device_ms = ms;
-    std::vector<MatResult> res;
-    int rc = r->set->read_results(res);
-    if (rc == GSOFA_OK) {
-      r->result = res[0];
-      const size_t n = (size_t)r->prep.n;
-      std::vector<int32_t> nzu(n);
-      std::vector<uint8_t> flags(n);
-      rc = r->set->read_counts(0, nullptr, nullptr, flags.data(), nzu.data());
-      if (rc == GSOFA_OK) {
-        r->nzu_digest = fnv1a(nzu.data(), n * 4);
-        r->sup_digest = fnv1a(flags.data(), n);
-      }
-    }
+    int rc = s.set->read_results(res);
     if (rc != GSOFA_OK) {
+      s.error = rc;
+      s.error_text = gsofa_last_error();
+    }
+  }
+  for (size_t i = 0; i < s.group.size(); ++i) {
+    Request* r = s.group[i];
+    r->error = s.error;
+    r->error_text = s.error_text;
+    if (r->error != GSOFA_OK) continue;
+    r->device_ms = ms;
+    r->result = res[i];
+    const size_t n = (size_t)r->prep.n;
+    std::vector<int32_t> nzu(n);
+    std::vector<uint8_t> flags(n);
+    const int rc = s.set->read_counts((int)i, nullptr, nullptr, flags.data(), nzu.data());
+    if (rc == GSOFA_OK) {
+      r->nzu_digest = fnv1a(nzu.data(), n * 4);
+      r->sup_digest = fnv1a(flags.data(), n);
+    } else {
       r->error = rc;
       r->error_text = gsofa_last_error();
     }
   }
-  r->set.reset();  // stream-ordered frees on the slot
-  r->t_done = Clock::now();
+  s.set.reset();  // stream-ordered frees on the slot
+  const Clock::time_point now = Clock::now();
   std::lock_guard<std::mutex> lk(mu);
-  in_use -= r->footprint;
-  --running;
-  ++completed;
-  if (r->cancel_requested) r->state = State::Cancelled;
-  else r->state = r->error == GSOFA_OK ? State::Done : State::Failed;
-  s.req = nullptr;
+  in_use -= s.footprint;
+  running -= (int)s.group.size();
+  completed += (int64_t)s.group.size();
+  for (Request* r : s.group) {
+    r->t_done = now;
+    if (r->cancel_requested) r->state = State::Cancelled;
+    else r->state = r->error == GSOFA_OK ? State::Done : State::Failed;
+  }
+  s.group.clear();
+  s.footprint = 0;
   cv_admit.notify_all();
   cv_done.notify_all();
 }
@@ -190,14 +236,15 @@ void gsofa_server::done_loop() {
       std::lock_guard<std::mutex> lk(mu);
       if (stopping && running == 0) return;
     }
-    for (auto& s : slots) {
-      Request* r;
+    for (auto& sp : slots) {
+      Slot& s = *sp;
+      bool busy;
       {
         std::lock_guard<std::mutex> lk(mu);
-        r = s.req;
+        busy = !s.group.empty();
       }
       // the admission thread records s.done after enqueueing; an unrecorded event reads as done
-      if (!r || !r->launched.load(std::memory_order_acquire)) continue;
+      if (!busy || !s.launched.load(std::memory_order_acquire)) continue;
       hipError_t q = hipEventQuery(s.done);
       if (q == hipErrorNotReady) continue;
       any = true;
@@ -214,6 +261,7 @@ void gsofa_server_options_default(gsofa_server_options* o) {
   std::memset(o, 0, sizeof(*o));
   o->slots = 4;
   o->hbm_budget_bytes = 0;  // 0 = 80% of free device memory at start
+  o->max_group = 16;
 }
 
 int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out) {
@@ -223,6 +271,7 @@ int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out) {
   gsofa_server_options_default(&def);
   if (!opt) opt = &def;
   if (opt->slots <= 0 || opt->slots > 64) return fail(GSOFA_ERR_INVALID_ARG, "slots must be in [1, 64]");
+  if (opt->max_group < 0 || opt->max_group > 1024) return fail(GSOFA_ERR_INVALID_ARG, "max_group must be in [0, 1024]");
   DeviceInfo info;
   int rc = device_info(opt->device, info);
   if (rc != GSOFA_OK) return rc;
@@ -231,13 +280,16 @@ int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out) {
   if (!srv) return fail(GSOFA_ERR_OUT_OF_MEMORY, "server");
   srv->device = opt->device;
   srv->cu_share = opt->cu_share < 0 ? 0 : opt->cu_share;
+  srv->max_group = opt->max_group > 0 ? opt->max_group : 16;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipMemGetInfo");
   srv->budget = opt->hbm_budget_bytes > 0 ? opt->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
-  srv->slots.resize(opt->slots);
-  for (auto& s : srv->slots) {
-    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&s.start) != hipSuccess || hipEventCreate(&s.done) != hipSuccess)
+  for (int k = 0; k < opt->slots; ++k) {
+    srv->slots.emplace_back(new (std::nothrow) Slot);
+    Slot* s = srv->slots.back().get();
+    if (!s) return fail(GSOFA_ERR_OUT_OF_MEMORY, "slot");
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->start) != hipSuccess || hipEventCreate(&s->done) != hipSuccess)
       return fail(GSOFA_ERR_HIP, "slot stream/event creation");
   }
   srv->admit_thread = std::thread([p = srv.get()] { p->admit_loop(); });
@@ -399,7 +451,8 @@ int gsofa_server_stop(gsofa_server* srv) {
   }
   if (srv->admit_thread.joinable()) srv->admit_thread.join();
   if (srv->done_thread.joinable()) srv->done_thread.join();  // drains running requests
-  for (auto& s : srv->slots) {
+  for (auto& sp : srv->slots) {
+    Slot& s = *sp;
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.start) (void)hipEventDestroy(s.start);
     if (s.done) (void)hipEventDestroy(s.done);

@@ -202,7 +202,9 @@ typedef struct gsofa_server_options {
   int32_t device;
   int32_t slots;            /* concurrent requests on the device, default 4                      */
   int64_t hbm_budget_bytes; /* admission budget; 0 = 80% of the free device memory at start      */
-  int32_t cu_share;         /* CUs each request's persistent grids are sized for; 0 = all        */
+  int32_t cu_share;         /* CUs each group's persistent grids are sized for; 0 = all          */
+  int32_t max_group;        /* requests a free slot takes from the queue at once (same options,
+                               FIFO order) and runs as one batch; default 16, 1 = one at a time */
 } gsofa_server_options;
 void gsofa_server_options_default(gsofa_server_options* opt);
 int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out);

@@ -60,6 +60,35 @@ def test_server_alone_and_in_the_mix():
         srv.stop()
 
 
+def test_server_groups_queued_requests():
+    """One slot: requests that queue behind a running one are taken as one group (one launch
+    sequence); each result equals the plan's, grouped or not."""
+    members = [6, 11, 2, 8, 9, 12]
+    pats = [g.generate_workload(g.WL_STREAM, m) if m % 2 == 0 else g.generate_workload(g.WL_BATCH256, m)
+            for m in members]
+    pats = [g.generate_workload(g.WL_GRID2D_659033)] + [p for p in pats if p.n <= 60000]  # the first one runs long
+    refs = [plan_reference(p) for p in pats]
+    for max_group in (1, 16):
+        srv = g.Server(slots=1, max_group=max_group)
+        try:
+            order = list(range(len(pats))) + list(range(1, len(pats)))
+            ids = [srv.submit(pats[i]) for i in order]
+            for i, rid in zip(order, ids):
+                r = srv.wait(rid, 300.0).as_dict()
+                ref, dig = refs[i]
+                assert all(r[key] == ref[key] for key in KEYS), (max_group, i)
+                info = srv.info(rid)
+                assert (info["nzu_digest"], info["supernode_digest"]) == dig
+            st = srv.stats()
+            say(f"[serve] max_group {max_group}: {st}")
+            if max_group == 1:
+                assert st["peak_running"] == 1
+            else:
+                assert st["peak_running"] >= 2  # a group formed on the single slot
+        finally:
+            srv.stop()
+
+
 def test_server_rejects_what_never_fits_and_cancels():
     small = g.generate_workload(g.WL_GRID2D_4096)
     srv = g.Server(slots=1, hbm_budget_bytes=64 << 20)
