@@ -49,7 +49,11 @@ def batched_vs_plain(g, members):
     run back to back. Per-member results must be identical."""
     pats = [g.generate_workload(g.WL_BATCH256, k) for k in range(members)]
     log(f"[batched] {members} members, n total {sum(p.n for p in pats)}")
-    b = g.Batch(pats)
+    t0 = time.perf_counter()
+    b = g.Batch(pats, autotune=1)  # shape and row-block size timed on this set (not in the timed region)
+    t_tune = time.perf_counter() - t0
+    cfg = b.config()
+    log(f"[batched] autotuned in {t_tune:.1f} s: {cfg}")
     b.run().sync()  # capture + warm-up
     reps = 2
     t0 = time.perf_counter()
@@ -72,7 +76,7 @@ def batched_vs_plain(g, members):
     log(f"[batched] batch {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, mismatches {mismatches}")
     return {"members": members, "batched_matrices_per_s": members / t_batch, "plain_matrices_per_s": members / t_plain,
             "speedup": t_plain / t_batch, "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
-            "bit_exact_vs_plain": mismatches == 0}
+            "bit_exact_vs_plain": mismatches == 0, "tuned_config": cfg, "autotune_s": t_tune}
 
 
 def served_vs_plain(g, nreq, rate, slots, cu_share=0):

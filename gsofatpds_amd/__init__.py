@@ -63,7 +63,7 @@ class Options(C.Structure):
     _fields_ = [("chunk_size", C.c_int32), ("n_processes", C.c_int32), ("structure", C.c_int32),
                 ("device", C.c_int32), ("heavy_threads", C.c_int32), ("heavy_wgs_per_cu", C.c_int32),
                 ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32), ("chain_block", C.c_int32),
-                ("pair_directions", C.c_int32)]
+                ("pair_directions", C.c_int32), ("autotune", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -86,6 +86,8 @@ _SIGNATURES = {
     "gsofa_host_matrix_free": (None, [C.c_void_p]),
     "gsofa_read_ascii": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "gsofa_write_ascii": (C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p, C.c_char_p]),
+    "gsofa_plan_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "gsofa_batch_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "gsofa_read_binary": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "gsofa_write_binary": (C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p, C.c_int32, C.c_int32]),
     "gsofa_symbfact_min_id": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_uint64),
@@ -381,6 +383,12 @@ class Plan:
         _check(lib().gsofa_work_split(self._h, C.byref(w), C.byref(r), C.byref(e)))
         return {"heavy_edge_checks": w.value, "heavy_reach": r.value, "heavy_entries": e.value}
 
+    def config(self) -> dict:
+        """Launch configuration in use (after any autotuning)."""
+        w = (C.c_int32 * 4)()
+        _check(lib().gsofa_plan_config(self._h, w))
+        return {"heavy_threads": w[0], "heavy_wgs_per_cu": w[1], "chain_block": w[2], "queue_capacity": w[3]}
+
     def executed_work(self) -> dict:
         w = (C.c_uint64 * 6)()
         _check(lib().gsofa_executed_work(self._h, w))
@@ -418,6 +426,12 @@ class Batch:
             self._h = None
 
     __del__ = close
+
+    def config(self) -> dict:
+        """Launch configuration in use (after any autotuning)."""
+        w = (C.c_int32 * 4)()
+        _check(lib().gsofa_batch_config(self._h, w))
+        return {"heavy_threads": w[0], "heavy_wgs_per_cu": w[1], "chain_block": w[2], "queue_capacity": w[3]}
 
     def run(self, stream=None) -> "Batch":
         _check(lib().gsofa_batch_run(self._h, _stream_handle(stream)))

@@ -27,6 +27,7 @@ Options to_options(const gsofa_options* o) {
   r.qcap = o->queue_capacity;
   r.chain_block = o->chain_block > 0 ? o->chain_block : 0;  // 0: chosen from n and the grid
   r.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
+  r.autotune = o->autotune ? 1 : 0;
   return r;
 }
 
@@ -41,6 +42,86 @@ void fill_result(const MatResult& r, const Options& opt, int32_t n, double ms, g
   out->supernodes = r.nsup + opt.n_processes;
   out->heavy_sources = r.heavy;
   out->time_ms = ms;
+}
+
+// One timed run of a candidate configuration on its own device set (a warm-up run first).
+int time_candidate(const std::vector<const Prepared*>& mats, const Options& o, double* ms, Options* chosen) {
+  std::unique_ptr<DeviceSet> set(new (std::nothrow) DeviceSet);
+  if (!set) return fail(GSOFA_ERR_OUT_OF_MEMORY, "device set");
+  int rc = set->create(mats, o);
+  if (rc != GSOFA_OK) return rc;
+  *chosen = set->options();
+  chosen->heavy_threads = set->heavy_threads();
+  chosen->heavy_wgs_per_cu = set->heavy_wgs_per_cu();
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = fail(GSOFA_ERR_HIP, "autotune stream/events");
+  if (rc == GSOFA_OK) rc = set->enqueue(s);
+  if (rc == GSOFA_OK && hipEventRecord(e0, s) != hipSuccess) rc = fail(GSOFA_ERR_HIP, "hipEventRecord");
+  if (rc == GSOFA_OK) rc = set->enqueue(s);
+  if (rc == GSOFA_OK && hipEventRecord(e1, s) != hipSuccess) rc = fail(GSOFA_ERR_HIP, "hipEventRecord");
+  float t = 0.f;
+  if (rc == GSOFA_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    rc = fail(GSOFA_ERR_HIP, "autotune timing");
+  if (s) (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  *ms = t;
+  return rc;
+}
+
+// The autotuner: bitmap-path workgroup shapes (threads x resident per CU: rows in flight against
+// frontier width per pass) at the automatic row-block size, then 2x and 4x larger blocks for the
+// winner (fewer chain starts, coarser balance). Candidates that do not fit the device are
+// skipped. Fields the caller set explicitly are kept. The structure is not produced while timing.
+int autotune(const std::vector<const Prepared*>& mats, const Options& base, Options& best) {
+  Options b = base;
+  b.structure = 0;
+  double best_ms = 1e300;
+  Options best_o = b;
+  bool any = false;
+  const int shapes[3][2] = {{256, 4}, {512, 2}, {1024, 1}};
+  for (const auto& sh : shapes) {
+    if (base.heavy_threads && base.heavy_threads != sh[0]) continue;
+    Options o = b;
+    o.heavy_threads = sh[0];
+    if (!base.heavy_wgs_per_cu) o.heavy_wgs_per_cu = sh[1];
+    double ms = 0.0;
+    Options chosen;
+    const int rc = time_candidate(mats, o, &ms, &chosen);
+    if (rc == GSOFA_ERR_CAPACITY) continue;
+    if (rc != GSOFA_OK) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_o = chosen;
+      any = true;
+    }
+  }
+  if (!any) return fail(GSOFA_ERR_CAPACITY, "autotune: no candidate fits the device");
+  if (base.chain_block == 0) {
+    const int32_t k0 = best_o.chain_block;
+    for (int mul : {2, 4}) {
+      Options o = best_o;
+      o.chain_block = std::min<int32_t>(512, k0 * mul);
+      if (o.chain_block <= k0) break;
+      double ms = 0.0;
+      Options chosen;
+      const int rc = time_candidate(mats, o, &ms, &chosen);
+      if (rc == GSOFA_ERR_CAPACITY) continue;
+      if (rc != GSOFA_OK) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_o = chosen;
+      }
+    }
+  }
+  best = best_o;
+  best.structure = base.structure;
+  best.autotune = 0;
+  return GSOFA_OK;
 }
 
 // Events and stream bookkeeping shared by plans and batches.
@@ -190,11 +271,17 @@ int gsofa_plan_create(const gsofa_csr* a, const gsofa_options* opt, gsofa_plan**
   *out = nullptr;
   gsofa_options def;
   gsofa_options_default(&def);
-  const Options o = to_options(opt ? opt : &def);
+  Options o = to_options(opt ? opt : &def);
   std::unique_ptr<gsofa_plan> p(new (std::nothrow) gsofa_plan);
   if (!p) return fail(GSOFA_ERR_OUT_OF_MEMORY, "plan");
   int rc = prepare(a, p->prep);
   if (rc != GSOFA_OK) return rc;
+  if (o.autotune) {
+    Options tuned;
+    rc = autotune({&p->prep}, o, tuned);
+    if (rc != GSOFA_OK) return rc;
+    o = tuned;
+  }
   rc = p->set.create({&p->prep}, o);
   if (rc != GSOFA_OK) return rc;
   rc = p->timing.init();
@@ -341,6 +428,25 @@ void gsofa_plan_destroy(gsofa_plan* p) { delete p; }
 
 // ---- batches ---------------------------------------------------------------------------------
 
+static void config_of(const DeviceSet& s, int32_t* out4) {
+  out4[0] = s.heavy_threads();
+  out4[1] = s.heavy_wgs_per_cu();
+  out4[2] = s.options().chain_block;
+  out4[3] = s.qcap();
+}
+
+int gsofa_plan_config(gsofa_plan* p, int32_t* out4) {
+  if (!p || !out4) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  config_of(p->set, out4);
+  return GSOFA_OK;
+}
+
+int gsofa_batch_config(gsofa_batch* b, int32_t* out4) {
+  if (!b || !out4) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  config_of(b->set, out4);
+  return GSOFA_OK;
+}
+
 int gsofa_batch_create(const gsofa_csr* mats, int32_t count, const gsofa_options* opt, gsofa_batch** out) {
   if (!out || !mats || count <= 0) return fail(GSOFA_ERR_INVALID_ARG, "bad batch arguments");
   *out = nullptr;
@@ -357,7 +463,14 @@ int gsofa_batch_create(const gsofa_csr* mats, int32_t count, const gsofa_options
     if (rc != GSOFA_OK) return fail(rc, "batch member " + std::to_string(i) + ": " + gsofa_last_error());
     ptrs[i] = &b->preps[i];
   }
-  int rc = b->set.create(ptrs, o);
+  int rc = GSOFA_OK;
+  if (o.autotune) {
+    Options tuned;
+    rc = autotune(ptrs, o, tuned);
+    if (rc != GSOFA_OK) return rc;
+    o = tuned;
+  }
+  rc = b->set.create(ptrs, o);
   if (rc != GSOFA_OK) return rc;
   rc = b->timing.init();
   if (rc != GSOFA_OK) return rc;

@@ -160,13 +160,22 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
 
   // ---- launch configuration ----
   bw_max_ = (n_max_ + 31) / 32;
-  heavy_threads_ = opt.heavy_threads ? opt.heavy_threads : (n_max_ > 150000 ? 1024 : 512);
-  if (heavy_threads_ != 256 && heavy_threads_ != 512 && heavy_threads_ != 1024)
-    return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
-  const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : (heavy_threads_ == 1024 ? 1 : 2);
-  const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
   const size_t bitmap_bytes = (size_t)bw_max_ * 4;
   const size_t reserve = 3072;  // static LDS of the kernel (2.2 KB) plus margin
+  // Workgroup size of the bitmap path. Registers allow 16 waves per CU whatever the size, so the
+  // choice trades rows in flight per CU (more, smaller workgroups) against frontier width per
+  // pass. Measured: unstructured sets (paired items) and small patterns run fastest as four
+  // 256-thread workgroups per CU, structured ones as two of 512 threads up to 150,000 rows,
+  // larger ones (whose bitmap leaves LDS for one) as one of 1024.
+  auto lds_fits = [&](int wgs) { return (size_t)wgs * (bitmap_bytes + reserve + 256 * 8) <= info.lds_per_cu; };
+  if (opt.heavy_threads) heavy_threads_ = opt.heavy_threads;
+  else if ((paired_any_ || n_max_ <= 16384) && lds_fits(4)) heavy_threads_ = 256;
+  else if (n_max_ <= 150000 && lds_fits(2)) heavy_threads_ = 512;
+  else heavy_threads_ = 1024;
+  if (heavy_threads_ != 256 && heavy_threads_ != 512 && heavy_threads_ != 1024)
+    return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
+  const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : 1024 / heavy_threads_;
+  const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
   if (bitmap_bytes + reserve + 256 * 8 > info.lds_per_block)
     return fail(GSOFA_ERR_CAPACITY, "n = " + std::to_string(n_max_) + " exceeds the LDS bitmap capacity of the device");
   if (opt.qcap) {
@@ -183,6 +192,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
   heavy_grid_ = cus * occ;
+  heavy_wgs_per_cu_ = occ;
   // Rows per workgroup-path item: long blocks chain more, short blocks keep every workgroup
   // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
   // (Measured on the unstructured batch members: paired items, which carry both directions,
@@ -321,7 +331,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
     paired |= pair_directions(*P, opt.pair_directions);
   }
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
-  const int64_t heavy_grid = (int64_t)cus * 2;  // upper bound on resident workgroups per CU
+  const int64_t heavy_grid = (int64_t)cus * 4;  // upper bound on resident workgroups per CU
   int64_t b = 0;
   b += 4 * (nrows + 1) * 3;           // rowptr A, rowptr A^T, cursor
   b += 4 * nnz * 2;                   // adj, adj^T

@@ -40,6 +40,7 @@ struct Options {
   int32_t qcap = 0;             // LDS queue entries per level buffer, 0 = tuned
   int32_t chain_block = 0;      // rows per workgroup-path item; 1 disables chaining; 0 = auto
   int32_t pair_directions = -1; // 1 on, 0 off, -1 where few rows hold their predecessor
+  int32_t autotune = 0;         // pick shape and block size by timing candidates on the set
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
 };
@@ -87,6 +88,7 @@ class DeviceSet {
   int heavy_grid() const { return heavy_grid_; }
   int light_grid() const { return light_grid_; }
   int qcap() const { return qcap_; }
+  int heavy_wgs_per_cu() const { return heavy_wgs_per_cu_; }
 
  private:
   template <typename T>
@@ -124,6 +126,7 @@ class DeviceSet {
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
   int64_t* blkq_base_d_ = nullptr;
   int32_t paired_any_ = 0;            // some matrix runs paired workgroup-path items
+  int heavy_wgs_per_cu_ = 0;          // resident bitmap-path workgroups per CU
   int32_t nb_max_ = 0;                // chain blocks of the largest matrix
   int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries

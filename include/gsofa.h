@@ -132,6 +132,9 @@ typedef struct gsofa_options {
   int32_t pair_directions;  /* workgroup-path items run both directions of a block and learn
                                s-1 in R_s from the other direction: 1 on, 0 off, -1 (default)
                                on where fewer than a quarter of the rows hold their predecessor */
+  int32_t autotune;         /* 1: at creation, time the bitmap-path workgroup shapes and larger
+                               row blocks on this very set and keep the fastest (results are
+                               identical under every candidate); 0 (default): built-in rules */
 } gsofa_options;
 
 typedef struct gsofa_result {
@@ -182,6 +185,10 @@ void gsofa_plan_destroy(gsofa_plan* plan);
 
 /* ---- batched form: many matrices in one captured graph of persistent launches ----------- */
 typedef struct gsofa_batch gsofa_batch;
+/* The launch configuration in use: out4 = {heavy_threads, bitmap workgroups per CU,
+ * chain_block, queue_capacity}. No counterpart in the reference. */
+int gsofa_plan_config(gsofa_plan* plan, int32_t* out4);
+int gsofa_batch_config(gsofa_batch* batch, int32_t* out4);
 int gsofa_batch_create(const gsofa_csr* mats, int32_t count, const gsofa_options* opt, gsofa_batch** out);
 /* Replays the batch's hipGraph on `hip_stream` (captured on the first call). */
 int gsofa_batch_run(gsofa_batch* batch, void* hip_stream);

@@ -218,6 +218,23 @@ def test_batch_equals_plain():
     b.close()
 
 
+def test_autotuned_batch_and_plan_are_identical():
+    """The autotuner only picks launch shapes: results equal the untuned ones."""
+    members = [6, 7, 11, 8, 2, 1]
+    pats = [g.generate_workload(g.WL_BATCH256, m) for m in members]
+    keys = ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes")
+    res = []
+    for opt in (dict(), dict(autotune=1)):
+        b = g.Batch(pats, **opt)
+        say(f"[autotune] batch {opt}: {b.config()}")
+        b.run().sync()
+        res.append([(tuple(getattr(b.result(i), k) for k in keys), b.nzu(i).tobytes(), b.supernode_flags(i).tobytes())
+                    for i in range(len(pats))])
+        b.close()
+    assert res[0] == res[1]
+    check(load_ds(), S=128, label="ds_11948 autotuned", autotune=1)
+
+
 def test_entry_and_cli_on_reference_files(tmp_path):
     ds = load_ds()
     files = [str(tmp_path / f) for f in ("colbeg_11948.dat", "colend_11948.dat", "csr_11948.dat")]
