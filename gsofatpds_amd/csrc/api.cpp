@@ -74,18 +74,20 @@ int time_candidate(const std::vector<const Prepared*>& mats, const Options& o, d
 }
 
 // The autotuner: bitmap-path workgroup shapes (threads x resident per CU: rows in flight against
-// frontier width per pass) at the automatic row-block size, then 2x and 4x larger blocks for the
-// winner (fewer chain starts, coarser balance). Candidates that do not fit the device are
-// skipped. Fields the caller set explicitly are kept. The structure is not produced while timing.
+// frontier width per pass and LDS queue depth) at the automatic row-block size, then 2x, 4x, ...
+// larger blocks for the winner (fewer chain starts, coarser balance) while they keep helping,
+// up to 1024 rows. Candidates that do not fit the device are skipped. Fields the caller set
+// explicitly are kept. The structure is not produced while timing.
 int autotune(const std::vector<const Prepared*>& mats, const Options& base, Options& best) {
   Options b = base;
   b.structure = 0;
   double best_ms = 1e300;
   Options best_o = b;
   bool any = false;
-  const int shapes[3][2] = {{256, 4}, {512, 2}, {1024, 1}};
+  const int shapes[4][2] = {{256, 4}, {256, 3}, {512, 2}, {1024, 1}};
   for (const auto& sh : shapes) {
     if (base.heavy_threads && base.heavy_threads != sh[0]) continue;
+    if (base.heavy_wgs_per_cu && sh[0] == 256 && sh[1] == 3) continue;  // same candidate twice
     Options o = b;
     o.heavy_threads = sh[0];
     if (!base.heavy_wgs_per_cu) o.heavy_wgs_per_cu = sh[1];
@@ -102,20 +104,18 @@ int autotune(const std::vector<const Prepared*>& mats, const Options& base, Opti
   }
   if (!any) return fail(GSOFA_ERR_CAPACITY, "autotune: no candidate fits the device");
   if (base.chain_block == 0) {
-    const int32_t k0 = best_o.chain_block;
-    for (int mul : {2, 4}) {
+    int32_t k = best_o.chain_block;
+    while (k < 1024) {
       Options o = best_o;
-      o.chain_block = std::min<int32_t>(512, k0 * mul);
-      if (o.chain_block <= k0) break;
+      o.chain_block = std::min<int32_t>(1024, 2 * k);
       double ms = 0.0;
       Options chosen;
       const int rc = time_candidate(mats, o, &ms, &chosen);
-      if (rc == GSOFA_ERR_CAPACITY) continue;
-      if (rc != GSOFA_OK) return rc;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best_o = chosen;
-      }
+      if (rc != GSOFA_OK && rc != GSOFA_ERR_CAPACITY) return rc;
+      if (rc != GSOFA_OK || ms >= best_ms) break;
+      best_ms = ms;
+      best_o = chosen;
+      k = o.chain_block;
     }
   }
   best = best_o;
