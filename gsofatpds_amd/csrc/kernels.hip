@@ -716,6 +716,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ unsigned long long s_wall, s_wovl, s_xe;
   __shared__ uint32_t s_rall, s_rovl;
   __shared__ int32_t s_deg0, s_import, s_src_was_set, s_early;
+  __shared__ uint32_t s_gap;  // X_t's vertices between the chain base t and src
   __shared__ uint32_t s_scan[kWaves];
   __shared__ int32_t s_esc[2][kHeavyWin + 1];   // row w0 + i was escalated (per direction)
   __shared__ int32_t s_hold[2][kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
@@ -767,13 +768,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
     // per direction (index = direction; a plain item uses its own direction's slot): the last
-    // row traversed in this block, whether its bitmap is saved, W_R, input degree, |R|, |U|,
-    // length of its staged U list; with paired items also the row before (pprev) and its save
-    int32_t prev0 = -1, prev1 = -1, pprev0 = -1, pprev1 = -1;
-    bool saved0 = false, saved1 = false, psaved0 = false, psaved1 = false;
+    // row traversed in this block, its W_R, input degree, |R|, |U| and staged U-list length, and
+    // which rows the two saved bitmaps of the direction hold (slot = row parity)
+    int32_t prev0 = -1, prev1 = -1;
+    int32_t sv00 = -1, sv01 = -1, sv10 = -1, sv11 = -1;  // sv<dir><parity>: row saved there
     uint64_t wrp0 = 0, wrp1 = 0, degp0 = 0, degp1 = 0, rp0 = 0, rp1 = 0;
     uint32_t up0 = 0, up1 = 0, ul0 = 0, ul1 = 0;
     int32_t lds_row = -1, lds_dir = -1;  // what the LDS bitmap holds
+#define GSOFA_SAVED_ROW(dd, r) \
+  ((r) >= 0 && ((dd) ? (((r) & 1) ? sv11 : sv10) : (((r) & 1) ? sv01 : sv00)) == (r))
     for (int32_t w0 = r0; w0 < r1; w0 += kHeavyWin) {
       const int32_t we = min(r1, w0 + kHeavyWin);
       const int32_t np = min(r1, we + 1) - w0;  // rows read: the window plus one look-ahead row
@@ -814,30 +817,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           if (s_esc[d][li] == 0) continue;  // the wave path finished this row
           const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
           int32_t* const ustage = d ? ustage1 : ustage0;
-          uint32_t* const gprev = gslot(d, src - 1);  // X_{src-1} of this direction, when saved
+          // The chain base t: the previous row of s-1 in this direction; with paired items the
+          // last row traversed here in this direction, even with wave-path rows in between.
           const int32_t prev_d = d ? prev1 : prev0;
-          const bool saved_d = d ? saved1 : saved0;
-          const bool follows = prev_d >= 0 && prev_d == src - 1;
-          const bool hold = s_hold[d][li] != 0;
-          const bool lds_prev = lds_row == src - 1 && lds_dir == d;  // LDS still holds X_{src-1}
+          const int32_t t = (prev_d >= 0 && (paired || prev_d == src - 1)) ? prev_d : -1;
+          uint32_t* const gprev = gslot(d, t);  // X_t of this direction, when saved
+          const bool saved_t = GSOFA_SAVED_ROW(d, t);
+          const bool lds_t = t >= 0 && lds_row == t && lds_dir == d;  // LDS still holds X_t
+          const bool hold = t == src - 1 && s_hold[d][li] != 0;         // row s holds t = s-1
 
-          const bool can_chain = follows && (lds_prev || saved_d);  // X_{src-1} is at hand
-          // ---- paired: is s-1 in R_s by the other direction's R'_{s-1}? ----
+          const bool can_chain = t >= 0 && (lds_t || saved_t);  // X_t is at hand
+          // ---- paired: is t in R_s, by the other direction's R'_t? An entry u of row s with
+          // u == t, or u < t in R'_t (u reaches t through vertices < t), puts t in R_s. ----
           bool early = false;
           if (paired && can_chain && !hold) {
             const int o = 1 - d;
-            // X'_{src-1} of the other direction: its last row, or (when that is src) the one before
-            const bool other_ok = ((o ? prev1 : prev0) == src - 1 && (o ? saved1 : saved0)) ||
-                                  ((o ? pprev1 : pprev0) == src - 1 && (o ? psaved1 : psaved0));
-            if (other_ok) {
-              const uint32_t* const gother = gslot(o, src - 1);
+            if (GSOFA_SAVED_ROW(o, t)) {
+              const uint32_t* const gother = gslot(o, t);
               if (wave == 0) {
                 const int32_t e = lane < kEll ? gload(M.ell[d] + (int64_t)src * kEll + lane) : kPad;
-                bool hit = e >= 0 && e < src - 1 && ((gload(gother + (e >> 5)) >> (e & 31)) & 1u) != 0;
+                bool hit = e == t || (e >= 0 && e < t && ((gload(gother + (e >> 5)) >> (e & 31)) & 1u) != 0);
                 if (lane == kEll - 1 && e <= -2) {
+                  hit = false;
                   for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !hit; ++p) {
                     const int32_t u = gload(M.adj[d] + p);
-                    hit = u < src - 1 && ((gload(gother + (u >> 5)) >> (u & 31)) & 1u) != 0;
+                    hit = u == t || (u < t && ((gload(gother + (u >> 5)) >> (u & 31)) & 1u) != 0);
                   }
                 }
                 const uint64_t b = __ballot(hit);
@@ -848,9 +852,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
           }
           const bool chained = can_chain && (hold || early);
-          const bool chain_lds = chained && lds_prev;
-          const bool chain_copy = chained && !lds_prev;
-          const bool has_prev = !chained && follows && saved_d;
+          const bool chain_lds = chained && lds_t;
+          const bool chain_copy = chained && !lds_t;
+          // late import (of X_{s-1} only, when s-1 is reached)
+          const bool has_prev = !chained && t == src - 1 && saved_t;
 
           // ---- start: bitmap, staged U list, counters ----
           if (!chained) {
@@ -863,7 +868,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
           } else {
             if (chain_copy) {
-              // X_{src-1} back from its saved copy, with the bit of src set
+              // X_t back from its saved copy, with the bit of src set
               for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
                 uint32_t pw[kImportBatch];
 #pragma unroll
@@ -919,6 +924,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             s_cnt[0] = 1;
             s_cnt[1] = 0;
             s_cnt[2] = 0;
+            s_gap = 0;
             s_unew = 0;
             s_useed = 0;
             s_wall = s_wovl = s_xe = 0;
@@ -927,6 +933,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             s_import = 0;
           }
           __syncthreads();
+          if (chained && t < src - 1) {
+            // X_t's vertices strictly between t and src are in R_s (reached through t's set) but
+            // their own neighbours are not covered: they start the first level after the seed.
+            const int32_t wlo = (t + 1) >> 5, whi = (src - 1) >> 5;
+            for (int32_t w = wlo + tid; w <= whi; w += kThreads) {
+              uint32_t x = bm[w];
+              const int32_t b0 = w * 32;
+              if (b0 <= t) x &= ~0u << (t - b0) << 1;                 // above t
+              if (b0 + 31 >= src) x &= (1u << (src - b0)) - 1u;      // below src (src > b0 here)
+              if (x) {
+                int32_t pos = atomicAdd(&s_cnt[1], (int32_t)__popc(x));
+                atomicAdd(&s_gap, (uint32_t)__popc(x));
+                while (x) {
+                  const int b = __ffs(x) - 1;
+                  x &= x - 1;
+                  qput(ql1, qg1, qcap, pos++, b0 + b);
+                }
+              }
+            }
+            __syncthreads();
+          }
 
           // this row's sums (each vertex is expanded at most once per row, so every sum stays
           // below the input nnz < 2^31)
@@ -1147,20 +1174,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
             U = s_unew;
           } else if (chained) {
+            // U_t's entries up to src (the gap vertices and src itself) are not U entries of src
             wr = wr_prev + deg_prev + s_wall;
             rr = r_prev + 1 + (uint64_t)s_rall;
-            U = u_prev - (uint32_t)s_src_was_set + s_unew;
+            U = u_prev - s_gap - (uint32_t)s_src_was_set + s_unew;
           } else {
             wr = s_wall;
             rr = s_rall;
             U = s_unew;
           }
-          // Save the bitmap for the next row: paired, whenever the next row is escalated in
-          // either direction (the LDS copy will not survive the other direction); plain, when
-          // the next row is escalated and does not hold this one (only an import can use it).
           const bool next_in = src + 1 < r1;
-          const bool save = paired ? next_in && (s_esc[0][li + 1] != 0 || s_esc[1][li + 1] != 0)
-                                   : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+          // Save the bitmap for later rows: paired, whenever a row follows in the block (it may
+          // chain from this one across wave-path rows, and the LDS copy will not survive the
+          // other direction); plain, when the next row is escalated and does not hold this one
+          // (only an import can use it).
+          const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
           const uint32_t ust_now = s_ust;
           const bool list_ok = want_list && s_lp[d][li + 1] - s_lp[d][li] == (int64_t)U && ust_now == U;
           if (tid == 0) {
@@ -1216,12 +1244,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             uint32_t* const gout = gslot(d, src);
             for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm[w]);
           }
+          if (save) {
+            const bool odd_row = (src & 1) != 0;
+            sv11 = (d && odd_row) ? src : sv11;
+            sv10 = (d && !odd_row) ? src : sv10;
+            sv01 = (!d && odd_row) ? src : sv01;
+            sv00 = (!d && !odd_row) ? src : sv00;
+          }
           if (d) {
-            pprev1 = prev1; psaved1 = saved1;
-            prev1 = src; saved1 = save; wrp1 = wr; degp1 = deg0; rp1 = rr; up1 = U; ul1 = ust_now;
+            prev1 = src; wrp1 = wr; degp1 = deg0; rp1 = rr; up1 = U; ul1 = ust_now;
           } else {
-            pprev0 = prev0; psaved0 = saved0;
-            prev0 = src; saved0 = save; wrp0 = wr; degp0 = deg0; rp0 = rr; up0 = U; ul0 = ust_now;
+            prev0 = src; wrp0 = wr; degp0 = deg0; rp0 = rr; up0 = U; ul0 = ust_now;
           }
           lds_row = src;
           lds_dir = d;
@@ -1233,6 +1266,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
 }
+#undef GSOFA_SAVED_ROW
 
 // ------------------------------------------------------------------------------------------
 // k_supernodes: the reference's rule (warpcentric_local_detection_test,
