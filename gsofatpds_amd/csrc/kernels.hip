@@ -323,6 +323,7 @@ constexpr int kLightLimit = 512;             // reached vertices before escalati
 constexpr int kLightStep = 4;                // ELL slots claimed together between escalation checks
 constexpr int kLightQ = kLightLimit + 64 * kLightStep;  // R entries never exceed the limit plus one step
 static_assert(kLightLimit + 64 * kLightStep < kHash, "the hash set must never fill");
+constexpr int kLightVpp = 64 / kEll;         // vertices of a narrow pass (one ELL slot per lane)
 
 __device__ __forceinline__ uint32_t hash_home(int32_t m) { return ((uint32_t)m * 0x9E3779B1u) >> (32 - kHashBits); }
 
@@ -399,13 +400,15 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
     for (int32_t src = s0; src < s1; ++src) {
       const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
       RowCursor rc;
-      load_row(M, dir, lane == 0 ? src : -1, rc);
-      // does row src hold src-1? (lane 0 owns the row)
+      // the source row, one ELL slot per lane (lanes 0..15); slot 15 <= -2 encodes a CSR tail
+      const int32_t sm = lane < kEll ? gload(M.ell[dir] + (int64_t)src * kEll + lane) : kPad;
+      // does row src hold src-1?
       bool has_prev_in_row = false;
-      if (lane == 0 && src > s0) {
-#pragma unroll
-        for (int j = 0; j < kEll; ++j) has_prev_in_row |= rc.e[j] == src - 1;
-        for (int32_t p = rc.ov_pos; p < rc.ov_end; ++p) has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
+      if (src > s0) {
+        has_prev_in_row = sm == src - 1;
+        if (sm <= -2)
+          for (int32_t p = -(sm + 2), pe = gload(M.rowptr[dir] + src + 1); p < pe; ++p)
+            has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
       const bool chained = __ballot(has_prev_in_row) != 0;
       bool esc = false;
@@ -423,13 +426,74 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
         src_was_set = __ballot(src_was_set) != 0;
         if (!src_was_set) ++hcount;
         int qn = 0;
-        uint32_t u_new = 0, u_seed = 0;
+        uint32_t u_new = 0, u_seed = 0;  // per lane, summed at the end of the row
         uint64_t w_new = 0, deg0 = 0;
         int head = 0;
-        bool seed = true;
-        while (true) {
+        // Narrow pass for the source row and for batches of at most kLightVpp queued vertices:
+        // lane = (vertex, ELL slot), one claim per lane, queue positions from the ballot rank.
+        // (The wide path below would run all its slot groups for one or two active lanes.)
+        auto claim_one = [&](int32_t m, bool seedpass) {
+          const bool valid = m >= 0;
+          const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+          const uint32_t nup = (uint32_t)__popcll(__ballot(valid && m > src));
+          if (seedpass) {
+            deg0 += nvalid;
+            if (lane == 0) u_seed += nup;  // entries of A's row above the diagonal
+          } else if (lane == 0) {
+            w_new += nvalid;
+          }
+          bool ins = false;
+          if (valid) {
+            const uint32_t home = hash_home(m);
+            const int32_t old = atomicCAS(&H[home], -1, m);
+            ins = old == -1 || (old != m && hash_probe(H, m, (home + 1) & (kHash - 1)) != 0);
+          }
+          const bool is_r = ins && m < src;
+          const uint64_t mr = __ballot(is_r);
+          if (is_r) {
+            const int pos = qn + (int)rank_below(mr);
+            if (pos < kLightQ) Q[pos] = m;
+          }
+          qn += __popcll(mr);
+          hcount += __popcll(__ballot(ins));
+          const uint32_t nu = (uint32_t)__popcll(__ballot(ins && m > src));
+          if (lane == 0) u_new += nu;
+          if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+        };
+        // m: this lane's slot of vertex v (slot 15 <= -2: the row continues in a CSR tail)
+        auto narrow = [&](int32_t v, int32_t m, bool seedpass) {
+          const bool tail = (lane & (kEll - 1)) == kEll - 1 && m <= -2;
+          claim_one(tail ? kPad : m, seedpass);
+          uint64_t tm = __ballot(tail);
+          while (tm != 0 && !esc) {
+            const int t = __ffsll((unsigned long long)tm) - 1;
+            tm &= tm - 1;
+            const int32_t vt = __builtin_amdgcn_readlane(v, t);
+            const int32_t a = -(__builtin_amdgcn_readlane(m, t) + 2);
+            const int32_t b = gload(M.rowptr[dir] + vt + 1);
+            for (int32_t p = a; p < b && !esc; p += 64) claim_one(p + lane < b ? gload(M.adj[dir] + p + lane) : kPad, seedpass);
+          }
+          if (dir == 0 && M.dup_all) {
+            const uint32_t dv = (v >= 0 && (lane & (kEll - 1)) == 0) ? (uint32_t)gload(M.dup_all + v) : 0u;
+            const uint32_t dsum = wave_sum_u32(dv);
+            if (seedpass) deg0 += dsum;
+            else if (lane == 0) w_new += dsum;
+          }
+        };
+        narrow(lane < kEll ? src : -1, sm, true);
+        bool seed = false;
+        while (!esc && head < qn) {
+          if (qn - head <= kLightVpp) {
+            const int vi = head + (lane >> 4);
+            const int32_t v = vi < qn ? Q[vi] : -1;
+            const int32_t m = v >= 0 ? gload(M.ell[dir] + (int64_t)v * kEll + (lane & (kEll - 1))) : kPad;
+            const int qend = qn;
+            narrow(v, m, false);
+            head = qend;
+            continue;
+          }
           int batch_end = 0;
-          if (!seed) {
+          {
             batch_end = min(head + 64, qn);
             const int32_t v = (head + lane < batch_end) ? Q[head + lane] : -1;
             load_row(M, dir, v, rc);
@@ -503,21 +567,13 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
             if (rc.ov_pos < rc.ov_end) m = gload(M.adj[dir] + rc.ov_pos++);
             visit(m);
           }
-          const int32_t vexp = seed ? (lane == 0 ? src : -1) : ((head + lane < batch_end) ? Q[head + lane] : -1);
+          const int32_t vexp = (head + lane < batch_end) ? Q[head + lane] : -1;
           if (vexp >= 0 && dir == 0 && M.dup_all) deg += gload(M.dup_all + vexp);
-          if (seed) deg0 = wave_sum_u32(deg);
-          else w_new += deg;
-          if (esc) break;
-          if (seed) {
-            seed = false;
-          } else {
-            head = batch_end;
-          }
-          if (head >= qn) break;
+          w_new += deg;
+          head = batch_end;
         }
         // per-row sums stay below the input nnz < 2^31
         const uint64_t w_new_all = wave_sum_u32((uint32_t)w_new);
-        deg0 = uniform_u64(deg0);
         x_edges += w_new_all + deg0;
         x_verts += (uint64_t)min(qn, kLightQ) + 1;
         if (!esc) {
