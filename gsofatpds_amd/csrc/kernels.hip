@@ -343,7 +343,7 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_
 // order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
 // X_s), and once such a chain escalates, its later rows escalate without an attempt.
 
-__global__ __launch_bounds__(kLightWaves * 64) void k_light(
+__global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_light(
     const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems) {
   __shared__ int32_t s_hash[kLightWaves][kHash];
   __shared__ int32_t s_q[kLightWaves][kLightQ];
@@ -390,11 +390,17 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
     const int32_t nblk = (M.n + kLightBlock - 1) / kLightBlock;
     const int32_t blk = nblk - 1 - (int32_t)(local >> 1);  // highest rows first
     const int32_t s0 = blk * kLightBlock, s1 = min(M.n, s0 + kLightBlock);
+    // the U list is kept where it is read: for the structure, and for direction 1's window bits
+    const bool track_u = M.list_ptr[dir] != nullptr || dir == 1;
 
     // state carried from row s-1 (all wave-uniform; hash H holds X_{s-1} when carry_ok)
     bool carry_ok = false;     // H holds X_{s-1}
     bool prev_esc = false;     // s-1 escalated
     int hcount = 0;            // entries of H
+    // The U list of the row in H (entries above its source) sits at the back of Q: Q[kLightQ-1-i],
+    // i < uq. New R entries fill Q from the front; both are disjoint subsets of H, whose size
+    // never exceeds kLightQ, so they cannot meet.
+    int uq = 0;
     uint32_t u_prev = 0;       // |U_{s-1}|
     uint64_t wr_prev = 0, deg_prev = 0, r_prev = 0;
     for (int32_t src = s0; src < s1; ++src) {
@@ -420,11 +426,25 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
           if (hcount > 0)
             for (int i = lane; i < kHash; i += 64) H[i] = -1;
           hcount = 0;
+          uq = 0;
         }
         bool src_was_set = false;
         if (lane == 0) src_was_set = hash_insert(H, src) == 0;
         src_was_set = __ballot(src_was_set) != 0;
         if (!src_was_set) ++hcount;
+        if (track_u && carry && src_was_set) {
+          // src was a U entry of src-1: drop it from the carried list (the last entry takes its place)
+          for (int base = 0; base < uq; base += 64) {
+            const int i = base + lane;
+            const uint64_t hit = __ballot(i < uq && Q[kLightQ - 1 - i] == src);
+            if (hit) {
+              const int at = base + __ffsll((unsigned long long)hit) - 1;
+              if (lane == 0) Q[kLightQ - 1 - at] = Q[kLightQ - uq];
+              break;
+            }
+          }
+          --uq;
+        }
         int qn = 0;
         uint32_t u_new = 0, u_seed = 0;  // per lane, summed at the end of the row
         uint64_t w_new = 0, deg0 = 0;
@@ -456,8 +476,13 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
           }
           qn += __popcll(mr);
           hcount += __popcll(__ballot(ins));
-          const uint32_t nu = (uint32_t)__popcll(__ballot(ins && m > src));
-          if (lane == 0) u_new += nu;
+          const bool is_u = ins && m > src;
+          const uint64_t mu = __ballot(is_u);
+          if (track_u) {
+            if (is_u) Q[kLightQ - 1 - (uq + (int)rank_below(mu))] = m;
+            uq += __popcll(mu);
+          }
+          if (lane == 0) u_new += (uint32_t)__popcll(mu);
           if (hcount > kLightLimit || qn >= kLightQ) esc = true;
         };
         // m: this lane's slot of vertex v (slot 15 <= -2: the row continues in a CSR tail)
@@ -517,7 +542,7 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
               home[j] = hash_home(m);
               old[j] = m >= 0 ? atomicCAS(&H[home[j]], -1, m) : m;
             }
-            uint32_t mr = 0, nnew = 0;
+            uint32_t mr = 0, mu = 0, nnew = 0;
 #pragma unroll
             for (int j = 0; j < kLightStep; ++j) {
               const int32_t m = rc.e[h + j];
@@ -527,10 +552,11 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
                 if (claim(m, old[j], home[j])) {
                   ++nnew;
                   if (m < src) mr |= 1u << j;
-                  else u_new += m > src;
+                  else if (m > src) mu |= 1u << j;
                 }
               }
             }
+            u_new += __popc(mu);
             const uint32_t c = __popc(mr);
             const uint32_t incl = wave_incl_scan_u32(c);
             int pos = qn + (int)(incl - c);
@@ -541,6 +567,18 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
                 ++pos;
               }
             qn += __builtin_amdgcn_readlane((int)incl, 63);
+            if (track_u) {
+              const uint32_t cu = __popc(mu);
+              const uint32_t inclu = wave_incl_scan_u32(cu);
+              int upos = uq + (int)(inclu - cu);
+#pragma unroll
+              for (int j = 0; j < kLightStep; ++j)
+                if ((mu >> j) & 1u) {
+                  if (upos < kLightQ) Q[kLightQ - 1 - upos] = rc.e[h + j];
+                  ++upos;
+                }
+              uq += __builtin_amdgcn_readlane((int)inclu, 63);
+            }
             hcount += (int)wave_sum_u32(nnew);
             if (hcount > kLightLimit || qn >= kLightQ) esc = true;
           }
@@ -559,6 +597,11 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
             }
             qn += __popcll(mr);
             hcount += __popcll(mn);
+            if (track_u) {
+              const uint64_t mu = __ballot(is_u);
+              if (is_u && uq + (int)rank_below(mu) < kLightQ) Q[kLightQ - 1 - (uq + (int)rank_below(mu))] = m;
+              uq += __popcll(mu);
+            }
             u_new += is_u;
             if (hcount > kLightLimit || qn >= kLightQ) esc = true;
           };
@@ -594,29 +637,19 @@ __global__ __launch_bounds__(kLightWaves * 64) void k_light(
           }
           if (dir == 1) {
             uint32_t win[4] = {0, 0, 0, 0};
-            for (int i = lane; i < kHash; i += 64) {
-              const int32_t m = H[i];
-              if (m > src) window_bit(win, src, m, chunk_end);
-            }
+            for (int i = lane; i < uq; i += 64) window_bit(win, src, Q[kLightQ - 1 - i], chunk_end);
             const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
             const uint32_t w2 = wave_or_u32(win[2]), w3 = wave_or_u32(win[3]);
             if (lane == 0) reinterpret_cast<uint4*>(M.win)[src] = make_uint4(w0, w1, w2, w3);
           }
-          // structure: the U entries (m > src) are the hash members above src
+          // structure: the U list at the back of Q
           if (M.list_ptr[dir] != nullptr) {
             const int64_t off = gload(M.list_ptr[dir] + src);
             const int64_t end = gload(M.list_ptr[dir] + src + 1);
-            if (end - off != (int64_t)U) {
+            if (end - off != (int64_t)U || uq != (int)U || qn + uq > kLightQ) {
               if (lane == 0) atomicOr(&M.acc[kAccOverflow], 1ull);
             } else {
-              int64_t wpos = off;
-              for (int i = lane; i < kHash; i += 64) {
-                const int32_t m = H[i];
-                const bool take = m > src;
-                const uint64_t mt = __ballot(take);
-                if (take) gstore(M.pool[dir] + wpos + rank_below(mt), m);
-                wpos += __popcll(mt);
-              }
+              for (int i = lane; i < uq; i += 64) gstore(M.pool[dir] + off + i, Q[kLightQ - 1 - i]);
             }
           }
           carry_ok = true;
