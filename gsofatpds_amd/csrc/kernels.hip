@@ -806,6 +806,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ uint32_t s_rall, s_rovl;
   __shared__ int32_t s_deg0, s_import, s_src_was_set, s_early;
   __shared__ uint32_t s_gap;  // X_t's vertices between the chain base t and src
+  __shared__ int32_t s_drop;  // src was found (and removed) in the carried U list
   __shared__ uint32_t s_scan[kWaves];
   __shared__ int32_t s_esc[2][kHeavyWin + 1];   // row w0 + i was escalated (per direction)
   __shared__ int32_t s_hold[2][kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
@@ -835,7 +836,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
   int last_mat = -1;
   __shared__ Totals tot;  // thread 0 only (in LDS: keeps the registers for the traversal)
-  if (tid == 0) tot = Totals{};
+  if (tid == 0) {
+    tot = Totals{};
+    s_drop = 0;
+  }
 
   const unsigned long long total = ws->heavy_count;
   while (true) {
@@ -947,14 +951,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           const bool has_prev = !chained && t == src - 1 && saved_t;
 
           // ---- start: bitmap, staged U list, counters ----
+          uint32_t kept = 0;  // the carried U list kept for this row (when filtered)
           if (!chained) {
             const int32_t sw = src >> 5;
             const uint32_t sb = 1u << (src & 31);
             for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
-            if (tid == 0) {
-              s_ust = 0;
-              s_src_was_set = 0;
-            }
+            if (tid == 0) s_src_was_set = 0;
           } else {
             if (chain_copy) {
               // X_t back from its saved copy, with the bit of src set
@@ -979,10 +981,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
               }
             }
-            if (want_list) {
+            if (want_list && t == src - 1) {
+              // every carried entry is above src-1: only src itself can drop out. The one
+              // thread that finds it moves the last entry into its place.
+              const uint32_t len = d ? ul1 : ul0;
+              for (uint32_t i = tid; i < len; i += kThreads)
+                if (gload(ustage + i) == src) {
+                  gstore(ustage + i, gload(ustage + len - 1));
+                  s_drop = 1;
+                }
+            } else if (want_list) {
               // keep the carried U entries above src, in place (reads of a round finish before its writes)
               const uint32_t len = d ? ul1 : ul0;
-              uint32_t kept = 0;
               for (uint32_t base = 0; base < len; base += kThreads) {
                 const uint32_t i = base + tid;
                 const int32_t m = i < len ? gload(ustage + i) : -1;
@@ -999,7 +1009,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 kept += all;
                 __syncthreads();
               }
-              if (tid == 0) s_ust = kept;
             }
             if (chain_lds && tid == 0) {
               const uint32_t sb = 1u << (src & 31);
@@ -1014,6 +1023,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             s_cnt[1] = 0;
             s_cnt[2] = 0;
             s_gap = 0;
+            s_ust = 0;  // new U entries of this row; they go after the carried ones (ubase)
             s_unew = 0;
             s_useed = 0;
             s_wall = s_wovl = s_xe = 0;
@@ -1022,6 +1032,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             s_import = 0;
           }
           __syncthreads();
+          // the carried part of this row's U list (new entries are appended after it)
+          const uint32_t ubase = !(chained && want_list) ? 0u
+                                 : (t == src - 1 ? (d ? ul1 : ul0) - (uint32_t)s_drop : kept);
           if (chained && t < src - 1) {
             // X_t's vertices strictly between t and src are in R_s (reached through t's set) but
             // their own neighbours are not covered: they start the first level after the seed.
@@ -1130,7 +1143,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 if (tot_w != 0) {
                   uint32_t basepos = 0;
                   if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
-                  uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
+                  uint32_t pos = ubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
 #pragma unroll
                   for (int j = 0; j < kEll; ++j)
                     if ((mask_u >> j) & 1u) gstore(ustage + pos++, rc.e[j]);
@@ -1168,7 +1181,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     const int leader = __ffsll((unsigned long long)mu) - 1;
                     uint32_t basepos = 0;
                     if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
-                    basepos = __shfl(basepos, leader, 64);
+                    basepos = ubase + __shfl(basepos, leader, 64);
                     if (is_u) gstore(ustage + basepos + rank_below(mu), m);
                   }
                 }
@@ -1222,7 +1235,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
                     uint32_t base = 0;
                     if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
-                    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+                    base = ubase + (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
                     uint32_t pos = base + incl - cnt;
                     while (nb) {
                       const int b = __ffs(nb) - 1;
@@ -1278,9 +1291,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           // other direction); plain, when the next row is escalated and does not hold this one
           // (only an import can use it).
           const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
-          const uint32_t ust_now = s_ust;
+          const uint32_t ust_now = ubase + s_ust;  // this row's list length
           const bool list_ok = want_list && s_lp[d][li + 1] - s_lp[d][li] == (int64_t)U && ust_now == U;
           if (tid == 0) {
+            s_drop = 0;  // read into ubase by every thread before the barrier above
             if (mb != last_mat) {
               flush_totals(mats, last_mat, tot);
               last_mat = mb;
