@@ -234,8 +234,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(heavy_items_, 2 * nblk_total_, "escalation list"));
   // per resident workgroup: two frontier spill queues, the U list and the saved bitmap of the
   // last row (with paired items: a U list per direction, a bitmap per direction and row parity)
-  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * ((paired_any_ ? 4 : 3) * (size_t)n_max_ +
-                                                 (paired_any_ ? 4 : 1) * (((size_t)n_max_ + 31) / 32)),
+  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (size_t)heavy_region_ints(n_max_, paired_any_ != 0),
                   "heavy spill regions"));
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
@@ -339,7 +338,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
   b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
-  b += 4 * heavy_grid * ((paired ? 4 : 3) * (int64_t)nmax + (paired ? 4 : 1) * (((int64_t)nmax + 31) / 32));  // spill
+  b += 4 * heavy_grid * heavy_region_ints(nmax, paired);  // spill
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
   b += 1 << 20;                       // tables, accumulators, scan scratch

@@ -96,8 +96,15 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
                               int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s);
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap);
 hipError_t heavy_set_lds_limit(int threads, size_t bytes);
-// spill: per resident workgroup (3 n_max + ceil(n_max / 32)) ints, or (4 n_max + 4 ceil(n_max / 32))
-// when paired_any (some matrix of the set runs paired items).
+// Saved bitmaps per direction in a workgroup region when some matrix runs paired items: a ring
+// of the last rows saved (chain bases for later rows, early tests for the other direction).
+constexpr int kSavedRing = 3;
+// spill: ints per resident workgroup, (3 n_max + ceil(n_max / 32)), or with paired items
+// (4 n_max + 2 kSavedRing ceil(n_max / 32)).
+constexpr int64_t heavy_region_ints(int64_t n_max, bool paired_any) {
+  const int64_t nbw = (n_max + 31) / 32;
+  return paired_any ? 4 * n_max + 2 * kSavedRing * nbw : 3 * n_max + nbw;
+}
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);

@@ -737,6 +737,12 @@ __device__ __forceinline__ void qput(int32_t* l, int32_t* g, int32_t qcap, int32
   else gstore(g + (e - qcap), v);
 }
 
+// A row traversed by the bitmap path: W_R, input degree, |R|, |U| and its staged U-list length.
+struct RowFacts {
+  uint64_t wr, deg, r;
+  uint32_t u, ul;
+};
+
 struct Totals {  // cleared with Totals{}
   uint64_t fill, w, r, nnz_u, nnz_l, heavy;
   uint64_t x_edges, x_verts, x_out;
@@ -819,20 +825,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
   int32_t* const ql1 = ql0 + qcap;
   // Global region of this workgroup: two frontier spill queues, the U list of the last row of
-  // each direction, saved bitmaps: one, or with paired items four (direction x row parity, so
-  // that X_{s-1} of direction 0 survives while row s of direction 0 is saved).
+  // each direction, saved bitmaps: one, or with paired items a ring of the last kSavedRing rows
+  // saved per direction (chain bases, and the other direction's early tests).
+  constexpr int kRing = kPairedAny ? kSavedRing : 1;
   const int64_t nbw = (n_max + 31) / 32;
-  const int64_t region_len = (kPairedAny ? 4 : 3) * n_max + (kPairedAny ? 4 : 1) * nbw;
+  const int64_t region_len = heavy_region_ints(n_max, kPairedAny);
   int32_t* const region = spill_all + (int64_t)blockIdx.x * region_len;
   int32_t* const qg0 = region;
   int32_t* const qg1 = region + n_max;
   int32_t* const ustage0 = region + 2 * n_max;
   int32_t* const ustage1 = kPairedAny ? region + 3 * n_max : ustage0;
   uint32_t* const gsave = reinterpret_cast<uint32_t*>(region + (kPairedAny ? 4 : 3) * n_max);
-  // the saved bitmap of (row r, direction d)
-  auto gslot = [&](int dd, int32_t r) -> uint32_t* {
-    return kPairedAny ? gsave + (int64_t)(2 * dd + (r & 1)) * nbw : gsave;
+  // saved bitmap k of direction dd
+  auto gslot = [&](int dd, int k) -> uint32_t* {
+    return kPairedAny ? gsave + (int64_t)(kSavedRing * dd + k) * nbw : gsave;
   };
+  // Per direction: the rows of the saved bitmaps (index k < kSavedRing) and the last row
+  // traversed (index kSavedRing). Written by thread 0 after a row's last barrier.
+  __shared__ RowFacts s_facts[2][kSavedRing + 1];
 
   int last_mat = -1;
   __shared__ Totals tot;  // thread 0 only (in LDS: keeps the registers for the traversal)
@@ -864,12 +874,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     // row traversed in this block, its W_R, input degree, |R|, |U| and staged U-list length, and
     // which rows the two saved bitmaps of the direction hold (slot = row parity)
     int32_t prev0 = -1, prev1 = -1;
-    int32_t sv00 = -1, sv01 = -1, sv10 = -1, sv11 = -1;  // sv<dir><parity>: row saved there
-    uint64_t wrp0 = 0, wrp1 = 0, degp0 = 0, degp1 = 0, rp0 = 0, rp1 = 0;
-    uint32_t up0 = 0, up1 = 0, ul0 = 0, ul1 = 0;
+    int32_t sv00 = -1, sv01 = -1, sv02 = -1, sv10 = -1, sv11 = -1, sv12 = -1;  // sv<dir><k>: row in slot k
+    int wp0 = 0, wp1 = 0;                // the slot the next save of the direction takes
     int32_t lds_row = -1, lds_dir = -1;  // what the LDS bitmap holds
-#define GSOFA_SAVED_ROW(dd, r) \
-  ((r) >= 0 && ((dd) ? (((r) & 1) ? sv11 : sv10) : (((r) & 1) ? sv01 : sv00)) == (r))
+#define GSOFA_SV(dd, k) ((dd) ? ((k) == 0 ? sv10 : (k) == 1 ? sv11 : sv12) : ((k) == 0 ? sv00 : (k) == 1 ? sv01 : sv02))
+#define GSOFA_SLOT(dd, r)                                                             \
+  ((r) < 0 ? -1                                                                       \
+   : (dd)  ? (sv10 == (r) ? 0 : (kRing > 1 && sv11 == (r)) ? 1 : (kRing > 2 && sv12 == (r)) ? 2 : -1) \
+           : (sv00 == (r) ? 0 : (kRing > 1 && sv01 == (r)) ? 1 : (kRing > 2 && sv02 == (r)) ? 2 : -1))
     for (int32_t w0 = r0; w0 < r1; w0 += kHeavyWin) {
       const int32_t we = min(r1, w0 + kHeavyWin);
       const int32_t np = min(r1, we + 1) - w0;  // rows read: the window plus one look-ahead row
@@ -910,45 +922,76 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           if (s_esc[d][li] == 0) continue;  // the wave path finished this row
           const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
           int32_t* const ustage = d ? ustage1 : ustage0;
-          // The chain base t: the previous row of s-1 in this direction; with paired items the
-          // last row traversed here in this direction, even with wave-path rows in between.
           const int32_t prev_d = d ? prev1 : prev0;
-          const int32_t t = (prev_d >= 0 && (paired || prev_d == src - 1)) ? prev_d : -1;
-          uint32_t* const gprev = gslot(d, t);  // X_t of this direction, when saved
-          const bool saved_t = GSOFA_SAVED_ROW(d, t);
-          const bool lds_t = t >= 0 && lds_row == t && lds_dir == d;  // LDS still holds X_t
-          const bool hold = t == src - 1 && s_hold[d][li] != 0;         // row s holds t = s-1
-
-          const bool can_chain = t >= 0 && (lds_t || saved_t);  // X_t is at hand
-          // ---- paired: is t in R_s, by the other direction's R'_t? An entry u of row s with
-          // u == t, or u < t in R'_t (u reaches t through vertices < t), puts t in R_s. ----
-          bool early = false;
-          if (paired && can_chain && !hold) {
+          // t1: the last row traversed here in this direction, even with wave-path rows in
+          // between (plain items: s-1 only)
+          const int32_t t1 = (prev_d >= 0 && (paired || prev_d == src - 1)) ? prev_d : -1;
+          const int slot1 = GSOFA_SLOT(d, t1);
+          const bool lds_t1 = t1 >= 0 && lds_row == t1 && lds_dir == d;  // LDS still holds X_t1
+          const bool hold = t1 == src - 1 && s_hold[d][li] != 0;           // row s holds t1 = s-1
+          // The chain base t (some t in R_s: X_s contains X_t) and the index of its facts.
+          int32_t t = t1;
+          int fi = kSavedRing;
+          bool chained = hold && (lds_t1 || slot1 >= 0);
+          if (kPairedAny && paired && !chained) {
+            // ---- paired: early tests of the saved rows c of this direction whose other
+            // direction is saved too. An entry u of row s with u == c, or u < c in R'_c (u reaches
+            // c through vertices < c), puts c in R_s. With a staged U list only t1 qualifies
+            // (the list is t1's). ----
             const int o = 1 - d;
-            if (GSOFA_SAVED_ROW(o, t)) {
-              const uint32_t* const gother = gslot(o, t);
+            int32_t c0 = GSOFA_SV(d, 0), c1 = GSOFA_SV(d, 1), c2 = GSOFA_SV(d, 2);
+            if (want_list) {
+              c0 = c0 == t1 ? c0 : -1;
+              c1 = c1 == t1 ? c1 : -1;
+              c2 = c2 == t1 ? c2 : -1;
+            }
+            const int o0 = GSOFA_SLOT(o, c0), o1 = GSOFA_SLOT(o, c1), o2 = GSOFA_SLOT(o, c2);
+            if (o0 >= 0 || o1 >= 0 || o2 >= 0) {
               if (wave == 0) {
+                const uint32_t* const g0 = gslot(o, o0 < 0 ? 0 : o0);
+                const uint32_t* const g1 = gslot(o, o1 < 0 ? 0 : o1);
+                const uint32_t* const g2 = gslot(o, o2 < 0 ? 0 : o2);
+                auto in_other = [&](int32_t u, int32_t c, int oc, const uint32_t* g) -> bool {
+                  return oc >= 0 && (u == c || (u >= 0 && u < c && ((gload(g + (u >> 5)) >> (u & 31)) & 1u) != 0));
+                };
                 const int32_t e = lane < kEll ? gload(M.ell[d] + (int64_t)src * kEll + lane) : kPad;
-                bool hit = e == t || (e >= 0 && e < t && ((gload(gother + (e >> 5)) >> (e & 31)) & 1u) != 0);
+                bool h0 = in_other(e, c0, o0, g0), h1 = in_other(e, c1, o1, g1), h2 = in_other(e, c2, o2, g2);
                 if (lane == kEll - 1 && e <= -2) {
-                  hit = false;
-                  for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !hit; ++p) {
+                  h0 = h1 = h2 = false;
+                  for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !(h0 && h1 && h2); ++p) {
                     const int32_t u = gload(M.adj[d] + p);
-                    hit = u == t || (u < t && ((gload(gother + (u >> 5)) >> (u & 31)) & 1u) != 0);
+                    h0 = h0 || in_other(u, c0, o0, g0);
+                    h1 = h1 || in_other(u, c1, o1, g1);
+                    h2 = h2 || in_other(u, c2, o2, g2);
                   }
                 }
-                const uint64_t b = __ballot(hit);
-                if (lane == 0) s_early = b != 0;
+                const uint64_t b0 = __ballot(h0), b1 = __ballot(h1), b2 = __ballot(h2);
+                if (lane == 0) s_early = (b0 != 0 ? 1 : 0) | (b1 != 0 ? 2 : 0) | (b2 != 0 ? 4 : 0);
               }
               __syncthreads();
-              early = s_early != 0;
+              // of the rows found in R_s, the one with the largest R (fewest vertices left)
+              const int hits = s_early;
+              int best = -1;
+              uint64_t best_r = 0;
+              for (int k = 0; k < kRing; ++k)
+                if (((hits >> k) & 1) && (best < 0 || s_facts[d][k].r > best_r)) {
+                  best = k;
+                  best_r = s_facts[d][k].r;
+                }
+              if (best >= 0) {
+                chained = true;
+                t = GSOFA_SV(d, best);
+                fi = best;
+              }
             }
           }
-          const bool chained = can_chain && (hold || early);
-          const bool chain_lds = chained && lds_t;
-          const bool chain_copy = chained && !lds_t;
+          const bool chain_lds = chained && t == t1 && lds_t1;
+          const bool chain_copy = chained && !chain_lds;
+          // the saved bitmap of the chain base, or of t1 for a late import
+          const int tslot = chained ? GSOFA_SLOT(d, t) : slot1;
+          uint32_t* const gprev = gslot(d, tslot < 0 ? 0 : tslot);
           // late import (of X_{s-1} only, when s-1 is reached)
-          const bool has_prev = !chained && t == src - 1 && saved_t;
+          const bool has_prev = !chained && t1 == src - 1 && slot1 >= 0;
 
           // ---- start: bitmap, staged U list, counters ----
           uint32_t kept = 0;  // the carried U list kept for this row (when filtered)
@@ -984,7 +1027,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             if (want_list && t == src - 1) {
               // every carried entry is above src-1: only src itself can drop out. The one
               // thread that finds it moves the last entry into its place.
-              const uint32_t len = d ? ul1 : ul0;
+              const uint32_t len = s_facts[d][fi].ul;
               for (uint32_t i = tid; i < len; i += kThreads)
                 if (gload(ustage + i) == src) {
                   gstore(ustage + i, gload(ustage + len - 1));
@@ -992,7 +1035,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
             } else if (want_list) {
               // keep the carried U entries above src, in place (reads of a round finish before its writes)
-              const uint32_t len = d ? ul1 : ul0;
+              const uint32_t len = s_facts[d][fi].ul;
               for (uint32_t base = 0; base < len; base += kThreads) {
                 const uint32_t i = base + tid;
                 const int32_t m = i < len ? gload(ustage + i) : -1;
@@ -1034,7 +1077,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           __syncthreads();
           // the carried part of this row's U list (new entries are appended after it)
           const uint32_t ubase = !(chained && want_list) ? 0u
-                                 : (t == src - 1 ? (d ? ul1 : ul0) - (uint32_t)s_drop : kept);
+                                 : (t == src - 1 ? s_facts[d][fi].ul - (uint32_t)s_drop : kept);
           if (chained && t < src - 1) {
             // X_t's vertices strictly between t and src are in R_s (reached through t's set) but
             // their own neighbours are not covered: they start the first level after the seed.
@@ -1265,10 +1308,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
             }
           }
+          // the base row's facts (of t when chained, of t1 for an import), read before thread 0
+          // writes this row's facts below
+          const RowFacts base = s_facts[d][fi];
           __syncthreads();
           const uint64_t deg0 = (uint64_t)s_deg0;
-          const uint64_t wr_prev = d ? wrp1 : wrp0, deg_prev = d ? degp1 : degp0, r_prev = d ? rp1 : rp0;
-          const uint32_t u_prev = d ? up1 : up0;
+          const uint64_t wr_prev = base.wr, deg_prev = base.deg, r_prev = base.r;
+          const uint32_t u_prev = base.u;
           uint64_t wr, rr;
           uint32_t U;
           if (imported) {
@@ -1293,7 +1339,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
           const uint32_t ust_now = ubase + s_ust;  // this row's list length
           const bool list_ok = want_list && s_lp[d][li + 1] - s_lp[d][li] == (int64_t)U && ust_now == U;
+          const int wslot = d ? wp1 : wp0;  // the ring slot this row's save takes
           if (tid == 0) {
+            // read by later rows after the barrier that ends this row
+            const RowFacts f{wr, deg0, rr, U, ust_now};
+            s_facts[d][kSavedRing] = f;
+            if (save) s_facts[d][wslot] = f;
             s_drop = 0;  // read into ubase by every thread before the barrier above
             if (mb != last_mat) {
               flush_totals(mats, last_mat, tot);
@@ -1344,21 +1395,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             for (; i < U; i += kThreads) gstore(dst + i, gload(ustage + i));
           }
           if (save) {
-            uint32_t* const gout = gslot(d, src);
+            uint32_t* const gout = gslot(d, wslot);
             for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm[w]);
+            sv12 = (d && wslot == 2) ? src : sv12;
+            sv11 = (d && wslot == 1) ? src : sv11;
+            sv10 = (d && wslot == 0) ? src : sv10;
+            sv02 = (!d && wslot == 2) ? src : sv02;
+            sv01 = (!d && wslot == 1) ? src : sv01;
+            sv00 = (!d && wslot == 0) ? src : sv00;
+            const int nxt = wslot + 1 == kRing ? 0 : wslot + 1;
+            wp1 = d ? nxt : wp1;
+            wp0 = d ? wp0 : nxt;
           }
-          if (save) {
-            const bool odd_row = (src & 1) != 0;
-            sv11 = (d && odd_row) ? src : sv11;
-            sv10 = (d && !odd_row) ? src : sv10;
-            sv01 = (!d && odd_row) ? src : sv01;
-            sv00 = (!d && !odd_row) ? src : sv00;
-          }
-          if (d) {
-            prev1 = src; wrp1 = wr; degp1 = deg0; rp1 = rr; up1 = U; ul1 = ust_now;
-          } else {
-            prev0 = src; wrp0 = wr; degp0 = deg0; rp0 = rr; up0 = U; ul0 = ust_now;
-          }
+          if (d) prev1 = src; else prev0 = src;
           lds_row = src;
           lds_dir = d;
           __syncthreads();
@@ -1369,7 +1418,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
 }
-#undef GSOFA_SAVED_ROW
+#undef GSOFA_SLOT
+#undef GSOFA_SV
 
 // ------------------------------------------------------------------------------------------
 // k_supernodes: the reference's rule (warpcentric_local_detection_test,
