@@ -341,7 +341,8 @@ __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_
 
 // One wave per item = a block of kLightBlock consecutive rows of one direction, in increasing
 // order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
-// X_s), and once such a chain escalates, its later rows escalate without an attempt.
+// X_s). A row that holds an escalated u < s escalates without an attempt: X_u, already over the
+// limit, is part of X_s.
 
 __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_light(
     const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems) {
@@ -387,8 +388,11 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     const MatDev& M = mats[mb];
     const int64_t local = k - item_base[mb];
     const int dir = (int)(local & 1);
+    // Paired matrices (few rows hold their predecessor, escalations spread over the rows): lowest
+    // rows first, so that their escalations are seen by the rows above. Otherwise highest rows
+    // first (the big separator rows start early).
     const int32_t nblk = (M.n + kLightBlock - 1) / kLightBlock;
-    const int32_t blk = nblk - 1 - (int32_t)(local >> 1);  // highest rows first
+    const int32_t blk = M.paired ? (int32_t)(local >> 1) : nblk - 1 - (int32_t)(local >> 1);
     const int32_t s0 = blk * kLightBlock, s1 = min(M.n, s0 + kLightBlock);
     // the U list is kept where it is read: for the structure, and for direction 1's window bits
     const bool track_u = M.list_ptr[dir] != nullptr || dir == 1;
@@ -417,6 +421,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
       const bool chained = __ballot(has_prev_in_row) != 0;
+      // an entry u < src (ELL slots) whose row escalated already: X_src contains X_u. A flag
+      // written by another wave may not be seen yet; that only costs the attempt. Read here,
+      // tested after the source row's pass (which hides the load).
+      const bool held_esc_lane = sm >= 0 && sm < src && gload(M.esc[dir] + sm) != 0;
       bool esc = false;
       if (chained && prev_esc) {
         esc = true;  // X_src contains the escalated X_{src-1}
@@ -506,6 +514,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           }
         };
         narrow(lane < kEll ? src : -1, sm, true);
+        if (__ballot(held_esc_lane) != 0) esc = true;  // X_src contains an escalated X_u
         bool seed = false;
         while (!esc && head < qn) {
           if (qn - head <= kLightVpp) {

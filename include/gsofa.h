@@ -145,7 +145,8 @@ typedef struct gsofa_result {
   uint64_t reach;             /* sum_i |R_i|                                                  */
   int64_t supernodes;         /* reported count: counted + n_processes                       */
   int64_t supernodes_counted; /* supernodes counted by the reference rule                     */
-  uint64_t heavy_sources;     /* traversals that took the workgroup (bitmap) path              */
+  uint64_t heavy_sources;     /* traversals that took the workgroup (bitmap) path; may vary
+                                 between runs (escalation can follow other rows' outcomes)     */
   double time_ms;             /* device time of the last run                                   */
 } gsofa_result;
 
