@@ -36,6 +36,10 @@ def parse():
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the dominant kernel (optional)")
     ap.add_argument("--batch-members", type=int, default=256,
                     help="members of the 256-matrix batch to run batched vs plain (0 = skip)")
+    ap.add_argument("--batch-config", default="",
+                    help="heavy_threads,wgs_per_cu,chain_block for the batch instead of autotuning it")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only replay the batch (--steps replays after one warm-up), for kernel traces")
     ap.add_argument("--served-requests", type=int, default=32,
                     help="requests of the Poisson stream to serve (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
@@ -44,17 +48,29 @@ def parse():
     return ap.parse_args()
 
 
-def batched_vs_plain(g, members):
+def batched_vs_plain(g, members, config=None, only_replays=0):
     """The batch workload: batched (one captured graph for all members) vs one plan per member
     run back to back. Per-member results must be identical."""
     pats = [g.generate_workload(g.WL_BATCH256, k) for k in range(members)]
     log(f"[batched] {members} members, n total {sum(p.n for p in pats)}")
     t0 = time.perf_counter()
-    b = g.Batch(pats, autotune=1)  # shape and row-block size timed on this set (not in the timed region)
+    if config:
+        b = g.Batch(pats, heavy_threads=config[0], heavy_wgs_per_cu=config[1], chain_block=config[2])
+    else:
+        b = g.Batch(pats, autotune=1)  # shape and row-block size timed on this set (not in the timed region)
     t_tune = time.perf_counter() - t0
     cfg = b.config()
-    log(f"[batched] autotuned in {t_tune:.1f} s: {cfg}")
+    log(f"[batched] {'set created' if config else 'autotuned'} in {t_tune:.1f} s: {cfg}")
     b.run().sync()  # capture + warm-up
+    if only_replays:
+        t0 = time.perf_counter()
+        for _ in range(only_replays):
+            b.run().sync()
+        t_batch = (time.perf_counter() - t0) / only_replays
+        b.close()
+        log(f"[batched] {only_replays} replays, {t_batch * 1e3:.1f} ms each")
+        return {"members": members, "batched_ms": t_batch * 1e3, "batched_matrices_per_s": members / t_batch,
+                "config": cfg}
     reps = 2
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -165,6 +181,12 @@ def main():
         sys.exit(2)
     import gsofatpds_amd as g
 
+    batch_config = tuple(int(x) for x in args.batch_config.split(",")) if args.batch_config else None
+    if args.batch_only:
+        print(json.dumps({"batched": batched_vs_plain(g, args.batch_members, batch_config, max(args.steps, 1))}),
+              flush=True)
+        return
+
     wl = {"grid2d_659033": g.WL_GRID2D_659033, "grid3d_110592": g.WL_GRID3D_110592,
           "grid2d_4096": g.WL_GRID2D_4096}[args.workload]
     t0 = time.perf_counter()
@@ -242,7 +264,7 @@ def main():
     }
     plan.close()
     if args.batch_members > 0:
-        out["batched"] = batched_vs_plain(g, args.batch_members)
+        out["batched"] = batched_vs_plain(g, args.batch_members, batch_config)
     if args.served_requests > 0:
         out["served"] = served_vs_plain(g, args.served_requests, args.served_rate, args.slots, args.cu_share)
     if not args.no_cpu:
