@@ -197,9 +197,13 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
   // (Measured on the unstructured batch members: paired items, which carry both directions,
   // do best with the same rule; one block per workgroup left the tail to the costliest ones.)
+  // Capped at 64 rows, or 128 with one workgroup per CU or paired items: measured best at 128 on
+  // the 659,033-row stencil (1024 x 1), at 64 on the 110,592-row one (512 x 2, 128 rows left
+  // too few blocks), at 80 = n / (2 grid) on a 165,532-row random member.
   if (opt_.chain_block == 0) {
+    const int64_t cap = (heavy_wgs_per_cu_ == 1 || paired_any_) ? 128 : 64;
     int64_t k = (int64_t)n_max_ / std::max<int64_t>(1, 2 * (int64_t)heavy_grid_);
-    opt_.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(64, k));
+    opt_.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(cap, k));
   }
   blkq_base_h_.assign(nmats_ + 1, 0);
   for (int b = 0; b < nmats_; ++b)
