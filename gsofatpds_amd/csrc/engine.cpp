@@ -147,14 +147,12 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     n_max_ = std::max(n_max_, P.n);
     row_base_h_[b + 1] = row_base_h_[b] + P.n;
     nnz_base_h_[b + 1] = nnz_base_h_[b] + (int64_t)P.adj.size();
-    item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)P.n + kLightBlock - 1) / kLightBlock);
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
     paired_any_ |= pair_directions(P, opt.pair_directions) ? 1 : 0;
   }
   nrows_ = row_base_h_[nmats_];
   nnz_ = nnz_base_h_[nmats_];
-  nitems_ = item_base_h_[nmats_];
   nblocks_sup_ = blk_base_h_[nmats_];
   if (nnz_ > (int64_t)INT32_MAX - 64) return fail(GSOFA_ERR_CAPACITY, "total nnz of a set must stay below 2^31 - 64");
 
@@ -213,6 +211,14 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   const int occ_l = light_occupancy();
   if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
   light_grid_ = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
+  {
+    const int64_t waves = (int64_t)light_grid_ * kLightWaves;
+    light_block_ = kLightBlockMin;
+    while (light_block_ < kLightBlockMax && 2 * nrows_ / (2 * light_block_) >= 4 * waves) light_block_ *= 2;
+  }
+  for (int b = 0; b < nmats_; ++b)
+    item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)ns_[b] + light_block_ - 1) / light_block_);
+  nitems_ = item_base_h_[nmats_];
 
   // ---- allocations ----
   GSOFA_TRY(alloc(rowptr_a_, nrows_ + 1, "rowptr A"));
@@ -311,6 +317,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
     M.chain_block = opt_.chain_block;
     M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
+    M.light_block = light_block_;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");

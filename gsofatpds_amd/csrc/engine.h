@@ -127,6 +127,7 @@ class DeviceSet {
   int64_t* blkq_base_d_ = nullptr;
   int32_t paired_any_ = 0;            // some matrix runs paired workgroup-path items
   int heavy_wgs_per_cu_ = 0;          // resident bitmap-path workgroups per CU
+  int32_t light_block_ = kLightBlockMin;  // rows per wave-path item
   int32_t nb_max_ = 0;                // chain blocks of the largest matrix
   int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries

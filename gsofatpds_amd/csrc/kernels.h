@@ -56,6 +56,7 @@ struct MatDev {
   uint32_t* blk_flag[2];          // ceil(n / chain_block): 1 where the block has an escalated row
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
   int32_t paired;                 // 1: workgroup-path items run both directions of a block
+  int32_t light_block;            // rows per wave-path item
 };
 
 // Scheduler state of one launch set (plain matrix or batch).
@@ -109,7 +110,12 @@ hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
-constexpr int kLightBlock = 8;  // consecutive rows per wave-path item
+constexpr int kLightWaves = 4;  // wavefronts per wave-path workgroup
+// Consecutive rows per wave-path item: the largest power of two in [kLightBlockMin,
+// kLightBlockMax] that still leaves about four items per resident wavefront (longer blocks chain
+// more rows inside the wave path; measured on the 659,033-row stencil: 8 rows 2.52 ms, 16 rows
+// 2.35 ms, 64 rows 2.23 ms).
+constexpr int kLightBlockMin = 8, kLightBlockMax = 64;
 }  // namespace gsofa
 
 namespace gsofa {

@@ -316,7 +316,6 @@ __device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t
 // ------------------------------------------------------------------------------------------
 // k_light: one wave per source, LDS hash set of the reached vertices
 // ------------------------------------------------------------------------------------------
-constexpr int kLightWaves = 4;
 constexpr int kHashBits = 10;
 constexpr int kHash = 1 << kHashBits;
 constexpr int kLightLimit = 512;             // reached vertices before escalation
@@ -339,7 +338,7 @@ __device__ __forceinline__ int hash_probe(int32_t* H, int32_t m, uint32_t h) {
 }
 __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_probe(H, m, hash_home(m)); }
 
-// One wave per item = a block of kLightBlock consecutive rows of one direction, in increasing
+// One wave per item = a block of M.light_block consecutive rows of one direction, in increasing
 // order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
 // X_s). A row that holds an escalated u < s escalates without an attempt: X_u, already over the
 // limit, is part of X_s.
@@ -391,9 +390,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     // Paired matrices (few rows hold their predecessor, escalations spread over the rows): lowest
     // rows first, so that their escalations are seen by the rows above. Otherwise highest rows
     // first (the big separator rows start early).
-    const int32_t nblk = (M.n + kLightBlock - 1) / kLightBlock;
+    const int32_t lb = M.light_block;
+    const int32_t nblk = (M.n + lb - 1) / lb;
     const int32_t blk = M.paired ? (int32_t)(local >> 1) : nblk - 1 - (int32_t)(local >> 1);
-    const int32_t s0 = blk * kLightBlock, s1 = min(M.n, s0 + kLightBlock);
+    const int32_t s0 = blk * lb, s1 = min(M.n, s0 + lb);
     // the U list is kept where it is read: for the structure, and for direction 1's window bits
     const bool track_u = M.list_ptr[dir] != nullptr || dir == 1;
 
