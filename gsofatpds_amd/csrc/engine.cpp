@@ -318,6 +318,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.chain_block = opt_.chain_block;
     M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
     M.light_block = light_block_;
+    M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall : kLightLimitMax;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");

@@ -57,6 +57,7 @@ struct MatDev {
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
   int32_t paired;                 // 1: workgroup-path items run both directions of a block
   int32_t light_block;            // rows per wave-path item
+  int32_t light_limit;            // reached vertices before the wave path escalates a row
 };
 
 // Scheduler state of one launch set (plain matrix or batch).
@@ -116,6 +117,11 @@ constexpr int kLightWaves = 4;  // wavefronts per wave-path workgroup
 // more rows inside the wave path; measured on the 659,033-row stencil: 8 rows 2.52 ms, 16 rows
 // 2.35 ms, 64 rows 2.23 ms).
 constexpr int kLightBlockMin = 8, kLightBlockMax = 64;
+// Reached vertices before the wave path escalates a row: 256 for matrices up to 32,768 rows (a
+// bitmap-path row is cheap there), else 512 (the LDS sizing). Measured: 4,096 rows 1.43 -> 1.17 ms
+// with 256; 110,592 rows 2.89 ms with 512, 3.02 with 256; 659,033 rows 8.4 / 9.4 ms.
+constexpr int kLightLimitMax = 512;
+constexpr int kLightLimitSmall = 256, kLightSmallRows = 32768;
 }  // namespace gsofa
 
 namespace gsofa {

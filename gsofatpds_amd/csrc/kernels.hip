@@ -318,7 +318,7 @@ __device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t
 // ------------------------------------------------------------------------------------------
 constexpr int kHashBits = 10;
 constexpr int kHash = 1 << kHashBits;
-constexpr int kLightLimit = 512;             // reached vertices before escalation
+constexpr int kLightLimit = kLightLimitMax;  // LDS sizing: the largest escalation limit
 constexpr int kLightStep = 4;                // ELL slots claimed together between escalation checks
 constexpr int kLightQ = kLightLimit + 64 * kLightStep;  // R entries never exceed the limit plus one step
 static_assert(kLightLimit + 64 * kLightStep < kHash, "the hash set must never fill");
@@ -391,6 +391,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     // rows first, so that their escalations are seen by the rows above. Otherwise highest rows
     // first (the big separator rows start early).
     const int32_t lb = M.light_block;
+    const int limit = M.light_limit;  // reached vertices before escalation
     const int32_t nblk = (M.n + lb - 1) / lb;
     const int32_t blk = M.paired ? (int32_t)(local >> 1) : nblk - 1 - (int32_t)(local >> 1);
     const int32_t s0 = blk * lb, s1 = min(M.n, s0 + lb);
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             uq += __popcll(mu);
           }
           if (lane == 0) u_new += (uint32_t)__popcll(mu);
-          if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+          if (hcount > limit || qn >= kLightQ) esc = true;
         };
         // m: this lane's slot of vertex v (slot 15 <= -2: the row continues in a CSR tail)
         auto narrow = [&](int32_t v, int32_t m, bool seedpass) {
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
               uq += __builtin_amdgcn_readlane((int)inclu, 63);
             }
             hcount += (int)wave_sum_u32(nnew);
-            if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+            if (hcount > limit || qn >= kLightQ) esc = true;
           }
           auto visit = [&](int32_t m) {
             const bool active = m >= 0;
@@ -612,7 +613,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
               uq += __popcll(mu);
             }
             u_new += is_u;
-            if (hcount > kLightLimit || qn >= kLightQ) esc = true;
+            if (hcount > limit || qn >= kLightQ) esc = true;
           };
           while (!esc && __ballot(rc.ov_pos < rc.ov_end) != 0) {
             int32_t m = kPad;
