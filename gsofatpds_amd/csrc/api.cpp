@@ -28,6 +28,7 @@ Options to_options(const gsofa_options* o) {
   r.chain_block = o->chain_block > 0 ? o->chain_block : 0;  // 0: chosen from n and the grid
   r.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   r.autotune = o->autotune ? 1 : 0;
+  r.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
   return r;
 }
 

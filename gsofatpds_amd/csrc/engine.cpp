@@ -122,6 +122,13 @@ int DeviceSet::alloc(T*& p, size_t count, const char* what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
+// Whether the bitmap path keeps its bitmaps in HBM: when asked for, or when one bitmap does not
+// fit in a workgroup's LDS next to the static part and the smallest queues.
+static bool place_bitmaps_in_hbm(int32_t n_max, const Options& opt, const DeviceInfo& info) {
+  const size_t bitmap_bytes = (((size_t)n_max + 31) / 32) * 4;
+  return opt.bitmap_in_hbm == 1 || bitmap_bytes + kHeavyLdsReserve + 256 * 8 > info.lds_per_block;
+}
+
 int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream) {
   opt_ = opt;
   alloc_stream_ = alloc_stream;
@@ -158,8 +165,11 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
 
   // ---- launch configuration ----
   bw_max_ = (n_max_ + 31) / 32;
-  const size_t bitmap_bytes = (size_t)bw_max_ * 4;
-  const size_t reserve = 3072;  // static LDS of the kernel (2.2 KB) plus margin
+  const size_t reserve = kHeavyLdsReserve;
+  // Bitmaps in HBM when asked for, or when one does not fit in a workgroup's LDS next to the
+  // smallest queues (n above about 1.2 M rows): slower claims (L2 atomics), no size limit.
+  hbm_bitmap_ = place_bitmaps_in_hbm(n_max_, opt, info);
+  const size_t bitmap_bytes = hbm_bitmap_ ? 0 : (size_t)bw_max_ * 4;
   // Workgroup size of the bitmap path. Registers allow 16 waves per CU whatever the size, so the
   // choice trades rows in flight per CU (more, smaller workgroups) against frontier width per
   // pass. Measured: unstructured sets (paired items) and small patterns run fastest as four
@@ -167,15 +177,16 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // larger ones (whose bitmap leaves LDS for one) as one of 1024.
   auto lds_fits = [&](int wgs) { return (size_t)wgs * (bitmap_bytes + reserve + 256 * 8) <= info.lds_per_cu; };
   if (opt.heavy_threads) heavy_threads_ = opt.heavy_threads;
+  else if (hbm_bitmap_) heavy_threads_ = 1024;
   else if ((paired_any_ || n_max_ <= 16384) && lds_fits(4)) heavy_threads_ = 256;
   else if (n_max_ <= 150000 && lds_fits(2)) heavy_threads_ = 512;
   else heavy_threads_ = 1024;
   if (heavy_threads_ != 256 && heavy_threads_ != 512 && heavy_threads_ != 1024)
     return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
+  if (hbm_bitmap_ && heavy_threads_ != 1024)
+    return fail(GSOFA_ERR_CAPACITY, "bitmaps in HBM run 1024-thread workgroups only");
   const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : 1024 / heavy_threads_;
   const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
-  if (bitmap_bytes + reserve + 256 * 8 > info.lds_per_block)
-    return fail(GSOFA_ERR_CAPACITY, "n = " + std::to_string(n_max_) + " exceeds the LDS bitmap capacity of the device");
   if (opt.qcap) {
     qcap_ = opt.qcap;
   } else {
@@ -183,10 +194,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     q = std::max<int64_t>(256, std::min<int64_t>(16384, q));
     qcap_ = (int32_t)(q & ~int64_t(63));
   }
-  const size_t lds = heavy_lds_bytes(bw_max_, qcap_);
+  const size_t lds = heavy_lds_bytes(bw_max_, qcap_, hbm_bitmap_);
   if (lds + reserve > info.lds_per_block) return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
-  GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds), "hipFuncSetAttribute(k_heavy)");
-  const int occ = heavy_occupancy(heavy_threads_, lds);
+  GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds, hbm_bitmap_), "hipFuncSetAttribute(k_heavy)");
+  const int occ = heavy_occupancy(heavy_threads_, lds, hbm_bitmap_);
   if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
   const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
   heavy_grid_ = cus * occ;
@@ -246,6 +257,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // last row (with paired items: a U list per direction, a bitmap per direction and row parity)
   GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (size_t)heavy_region_ints(n_max_, paired_any_ != 0),
                   "heavy spill regions"));
+  if (hbm_bitmap_) GSOFA_TRY(alloc(hbm_bm_, (size_t)heavy_grid_ * (size_t)bw_max_, "heavy bitmaps"));
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
   GSOFA_TRY(alloc(blk_flag_, (size_t)(2 * nblk_total_), "block flags"));
@@ -351,6 +363,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
   b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
   b += 4 * heavy_grid * heavy_region_ints(nmax, paired);  // spill
+  if (place_bitmaps_in_hbm(nmax, opt, info)) b += 4 * (int64_t)cus * (((int64_t)nmax + 31) / 32);  // one per CU
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
   b += 1 << 20;                       // tables, accumulators, scan scratch
@@ -380,7 +393,7 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
             "workgroup-path queue");
   if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
   GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, paired_any_,
-                         spill_, s),
+                         spill_, hbm_bm_, s),
             "k_heavy");
   if (ev) GSOFA_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
   GSOFA_HIP(launch_supernodes(mats_d_, nmats_, blk_base_d_, nblocks_sup_, s), "k_supernodes");

@@ -41,6 +41,7 @@ struct Options {
   int32_t chain_block = 0;      // rows per workgroup-path item; 1 disables chaining; 0 = auto
   int32_t pair_directions = -1; // 1 on, 0 off, -1 where few rows hold their predecessor
   int32_t autotune = 0;         // pick shape and block size by timing candidates on the set
+  int32_t bitmap_in_hbm = 0;    // bitmap-path bitmaps in HBM: 1 always, 0 only when LDS is too small
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
 };
@@ -89,6 +90,7 @@ class DeviceSet {
   int light_grid() const { return light_grid_; }
   int qcap() const { return qcap_; }
   int heavy_wgs_per_cu() const { return heavy_wgs_per_cu_; }
+  bool bitmap_in_hbm() const { return hbm_bitmap_; }
 
  private:
   template <typename T>
@@ -130,6 +132,8 @@ class DeviceSet {
   int32_t light_block_ = kLightBlockMin;  // rows per wave-path item
   int32_t nb_max_ = 0;                // chain blocks of the largest matrix
   int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
+  bool hbm_bitmap_ = false;           // bitmap-path bitmaps in HBM (hbm_bm_), not LDS
+  uint32_t* hbm_bm_ = nullptr;        // heavy_grid * bw_max words
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries
   int32_t *pool_u_ = nullptr, *pool_l_ = nullptr;
   std::vector<int64_t> ptr_base_h_;                        // matrix b's offsets start at row_base + b

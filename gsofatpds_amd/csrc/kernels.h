@@ -96,8 +96,11 @@ hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base,
 hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats, const int64_t* blkq_base,
                               const uint32_t* blk_flag, int32_t nb_max, int32_t* bucket, int32_t* cursor,
                               int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s);
-size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap);
-hipError_t heavy_set_lds_limit(int threads, size_t bytes);
+// LDS of k_heavy outside its dynamic part (static shared state, 2.6 KB) plus margin.
+constexpr size_t kHeavyLdsReserve = 3072;
+// Dynamic LDS of k_heavy: the bitmap (unless it lives in HBM) and two frontier queue levels.
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap);
+hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap);
 // Saved bitmaps per direction in a workgroup region when some matrix runs paired items: a ring
 // of the last rows saved (chain bases for later rows, early tests for the other direction).
 constexpr int kSavedRing = 3;
@@ -107,8 +110,11 @@ constexpr int64_t heavy_region_ints(int64_t n_max, bool paired_any) {
   const int64_t nbw = (n_max + 31) / 32;
   return paired_any ? 4 * n_max + 2 * kSavedRing * nbw : 3 * n_max + nbw;
 }
+// hbm_bitmaps: null (bitmaps in LDS), or grid * bw_max words, one bitmap per workgroup
+// (1024-thread workgroups only).
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
-                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s);
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
+                        uint32_t* hbm_bitmaps, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
 constexpr int kLightWaves = 4;  // wavefronts per wave-path workgroup
@@ -126,7 +132,7 @@ constexpr int kLightLimitSmall = 256, kLightSmallRows = 32768;
 
 namespace gsofa {
 // Resident workgroups of k_heavy per CU for a block size and dynamic LDS size.
-int heavy_occupancy(int threads, size_t lds_bytes);
+int heavy_occupancy(int threads, size_t lds_bytes, bool hbm_bitmap);
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipStream_t s);
 int light_occupancy();
 }  // namespace gsofa

@@ -791,11 +791,15 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
 // s-1, plus the vertices expanded here that are not in X_{s-1}.
 //
 // Paired items (M.paired) run both directions of the block, row by row (s dir 0, s dir 1,
-// s+1 dir 0, ...), and save every row's bitmap. Then s-1 in R_s is often known before any
-// expansion: if an entry u < s-1 of row s (in this direction) lies in the other direction's
-// R'_{s-1}, u reaches s-1 through vertices < s-1, so s -> u -> ... -> s-1 stays below s.
+// s+1 dir 0, ...), and save every row's bitmap. Then c in R_s is often known before any
+// expansion, for the last rows c saved in this direction: if an entry u < c of row s lies in the
+// other direction's R'_c, u reaches c through vertices < c, so s -> u -> ... -> c stays below s.
 // That turns the expensive late imports of unstructured patterns (where row s rarely holds
 // s-1) into chaining. Plain items run one direction and save a bitmap only for an import.
+//
+// kHbmBitmap: the bitmap lives in a per-workgroup HBM region instead of LDS (matrices whose
+// n-bit bitmap does not fit in LDS). Its claims are then L2 atomics, its reads bypass L1, and a
+// wave's plain stores to it (clears, copies) are acknowledged by L2 before each barrier.
 //
 // The kernel is bound by dependent global-memory round trips, one workgroup per CU: the
 // per-row facts (escalated? holds its predecessor? structure offsets) are read for a window
@@ -805,12 +809,13 @@ constexpr int kHeavyWin = 64;
 constexpr int kImportBatch = 8;
 
 // kPairedAny: some matrix of the set runs paired items (otherwise that code is compiled out).
-template <int kThreads, bool kPairedAny>
+template <int kThreads, bool kPairedAny, bool kHbmBitmap>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_heavy(const MatDev* __restrict__ mats,
                                                     WorkState* __restrict__ ws,
                                                     const uint64_t* __restrict__ heavy_items,
                                                     int32_t bw_max, int32_t qcap, int64_t n_max,
-                                                    int32_t* __restrict__ spill_all) {
+                                                    int32_t* __restrict__ spill_all,
+                                                    uint32_t* __restrict__ hbm_bitmaps) {
   static_assert(kEll == 16, "the window scan maps one row to 16 lanes");
   extern __shared__ uint32_t dyn[];
   constexpr int kWaves = kThreads / 64;
@@ -831,8 +836,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid >> 6;
-  uint32_t* bm = dyn;
-  int32_t* const ql0 = reinterpret_cast<int32_t*>(dyn + bw_max);
+  // the n-bit bitmap: LDS, or this workgroup's bw_max words of hbm_bitmaps
+  uint32_t* const bm = kHbmBitmap ? hbm_bitmaps + (int64_t)blockIdx.x * bw_max : dyn;
+  int32_t* const ql0 = reinterpret_cast<int32_t*>(kHbmBitmap ? dyn : dyn + bw_max);
+  // HBM bitmap: its claims are L2 atomics, so its reads bypass L1 (agent-scope loads, never a
+  // stale line), and each wave's plain stores (clears, copies) are acknowledged by L2 before a
+  // barrier lets another wave's atomics at the same words.
+  auto bm_get = [&](int32_t w) -> uint32_t {
+    if constexpr (kHbmBitmap) return __hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return bm[w];
+  };
+  auto bsync = [&]() {
+    if constexpr (kHbmBitmap) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  };
   int32_t* const ql1 = ql0 + qcap;
   // Global region of this workgroup: two frontier spill queues, the U list of the last row of
   // each direction, saved bitmaps: one, or with paired items a ring of the last kSavedRing rows
@@ -864,7 +881,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const unsigned long long total = ws->heavy_count;
   while (true) {
     if (tid == 0) s_idx = atomicAdd(&ws->heavy_next, 1ull);
-    __syncthreads();
+    bsync();
     const unsigned long long idx = s_idx;
     if (idx >= total) break;
     const uint64_t item = heavy_items[idx];
@@ -923,7 +940,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (tid < np) s_esc[dd][tid] = esc_v;
         if (want_list && tid <= np) s_lp[dd][tid] = lp_v;
       }
-      __syncthreads();
+      bsync();
 
       for (int32_t src = w0; src < we; ++src) {
         const int32_t li = src - w0;
@@ -978,7 +995,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 const uint64_t b0 = __ballot(h0), b1 = __ballot(h1), b2 = __ballot(h2);
                 if (lane == 0) s_early = (b0 != 0 ? 1 : 0) | (b1 != 0 ? 2 : 0) | (b2 != 0 ? 4 : 0);
               }
-              __syncthreads();
+              bsync();
               // of the rows found in R_s, the one with the largest R (fewest vertices left)
               const int hits = s_early;
               int best = -1;
@@ -1052,7 +1069,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 const bool keep = m > src;
                 const uint64_t bal = __ballot(keep);
                 if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
-                __syncthreads();
+                bsync();
                 uint32_t before = 0, all = 0;
                 for (int w = 0; w < kWaves; ++w) {
                   if (w < wave) before += s_scan[w];
@@ -1060,14 +1077,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
                 if (keep) gstore(ustage + kept + before + rank_below(bal), m);
                 kept += all;
-                __syncthreads();
+                bsync();
               }
             }
             if (chain_lds && tid == 0) {
               const uint32_t sb = 1u << (src & 31);
-              const uint32_t old = bm[src >> 5];
+              uint32_t old;
+              if constexpr (kHbmBitmap) {
+                old = atomicOr(&bm[src >> 5], sb);
+              } else {
+                old = bm[src >> 5];
+                bm[src >> 5] = old | sb;
+              }
               s_src_was_set = (old & sb) != 0;
-              bm[src >> 5] = old | sb;
             }
           }
           if (tid == 0) {
@@ -1084,7 +1106,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             s_deg0 = 0;
             s_import = 0;
           }
-          __syncthreads();
+          bsync();
           // the carried part of this row's U list (new entries are appended after it)
           const uint32_t ubase = !(chained && want_list) ? 0u
                                  : (t == src - 1 ? s_facts[d][fi].ul - (uint32_t)s_drop : kept);
@@ -1093,7 +1115,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             // their own neighbours are not covered: they start the first level after the seed.
             const int32_t wlo = (t + 1) >> 5, whi = (src - 1) >> 5;
             for (int32_t w = wlo + tid; w <= whi; w += kThreads) {
-              uint32_t x = bm[w];
+              uint32_t x = bm_get(w);
               const int32_t b0 = w * 32;
               if (b0 <= t) x &= ~0u << (t - b0) << 1;                 // above t
               if (b0 + 31 >= src) x &= (1u << (src - b0)) - 1u;      // below src (src > b0 here)
@@ -1107,7 +1129,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
               }
             }
-            __syncthreads();
+            bsync();
           }
 
           // this row's sums (each vertex is expanded at most once per row, so every sum stays
@@ -1211,7 +1233,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 if (active) {
                   const uint32_t bit = 1u << (m & 31);
                   uint32_t* wp = &bm[m >> 5];
-                  if ((*wp & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+                  if ((bm_get(m >> 5) & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
                 }
                 if (isnew && has_prev && m == src - 1) {
                   s_import = 1;
@@ -1257,7 +1279,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
               }
             }
-            __syncthreads();
+            bsync();
             // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
             if (!imported && s_import != 0) {
               // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
@@ -1273,9 +1295,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   const int32_t w = wb + q * kThreads + tid;
                   uint32_t nb = 0;
                   if (w < bw) {
-                    const uint32_t cw = bm[w];
+                    uint32_t cw;
+                    if constexpr (kHbmBitmap) {
+                      cw = pw[q] ? atomicOr(&bm[w], pw[q]) : 0u;  // keeps the level's L2 claims
+                    } else {
+                      cw = bm[w];
+                      bm[w] = cw | pw[q];
+                    }
                     nb = pw[q] & ~cw;
-                    bm[w] = cw | pw[q];
                     const int32_t b0 = w * 32;
                     if (b0 + 31 <= src) nb = 0;
                     else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
@@ -1299,7 +1326,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
               }
               imported = true;
-              __syncthreads();
+              bsync();
             }
           }
 
@@ -1321,7 +1348,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           // the base row's facts (of t when chained, of t1 for an import), read before thread 0
           // writes this row's facts below
           const RowFacts base = s_facts[d][fi];
-          __syncthreads();
+          bsync();
           const uint64_t deg0 = (uint64_t)s_deg0;
           const uint64_t wr_prev = base.wr, deg_prev = base.deg, r_prev = base.r;
           const uint32_t u_prev = base.u;
@@ -1379,8 +1406,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 uint32_t word = 0;
                 if (lo < chunk_end) {
                   const int32_t wi = lo >> 5, sh = lo & 31;
-                  const uint32_t a = bm[wi];
-                  const uint32_t b = (wi + 1 < bw) ? bm[wi + 1] : 0u;
+                  const uint32_t a = bm_get(wi);
+                  const uint32_t b = (wi + 1 < bw) ? bm_get(wi + 1) : 0u;
                   word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
                   const int32_t valid = chunk_end - lo;  // rows of the chunk left
                   if (valid < 32) word &= (1u << valid) - 1u;
@@ -1406,7 +1433,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           }
           if (save) {
             uint32_t* const gout = gslot(d, wslot);
-            for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm[w]);
+            for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm_get(w));
             sv12 = (d && wslot == 2) ? src : sv12;
             sv11 = (d && wslot == 1) ? src : sv11;
             sv10 = (d && wslot == 0) ? src : sv10;
@@ -1420,10 +1447,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           if (d) prev1 = src; else prev0 = src;
           lds_row = src;
           lds_dir = d;
-          __syncthreads();
+          bsync();
         }
       }
-      __syncthreads();  // the next window's facts overwrite these
+      bsync();  // the next window's facts overwrite these
     }
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
@@ -1545,51 +1572,49 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
   return hipGetLastError();
 }
 
-size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap) { return (size_t)bw_max * 4 + (size_t)qcap * 8; }
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap) {
+  return (hbm_bitmap ? 0 : (size_t)bw_max * 4) + (size_t)qcap * 8;
+}
 
-template <bool kP>
-hipError_t heavy_set_lds_limit_t(int threads, size_t bytes) {
-  const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
-  switch (threads) {
-    case 1024: return hipFuncSetAttribute((const void*)k_heavy<1024, kP>, a, (int)bytes);
-    case 512: return hipFuncSetAttribute((const void*)k_heavy<512, kP>, a, (int)bytes);
-    case 256: return hipFuncSetAttribute((const void*)k_heavy<256, kP>, a, (int)bytes);
-    default: return hipErrorInvalidValue;
+// The instantiation for (threads, paired, bitmap in HBM); the HBM bitmap runs 1024-thread
+// workgroups only. nullptr: no such instantiation.
+template <bool kP, bool kH>
+const void* heavy_fn(int threads) {
+  if constexpr (kH) {
+    return threads == 1024 ? (const void*)k_heavy<1024, kP, true> : nullptr;
+  } else {
+    switch (threads) {
+      case 1024: return (const void*)k_heavy<1024, kP, false>;
+      case 512: return (const void*)k_heavy<512, kP, false>;
+      case 256: return (const void*)k_heavy<256, kP, false>;
+      default: return nullptr;
+    }
   }
 }
 
-hipError_t heavy_set_lds_limit(int threads, size_t bytes) {
-  const hipError_t e = heavy_set_lds_limit_t<false>(threads, bytes);
-  return e != hipSuccess ? e : heavy_set_lds_limit_t<true>(threads, bytes);
+const void* heavy_fn(int threads, bool paired, bool hbm) {
+  if (paired) return hbm ? heavy_fn<true, true>(threads) : heavy_fn<true, false>(threads);
+  return hbm ? heavy_fn<false, true>(threads) : heavy_fn<false, false>(threads);
 }
 
-template <bool kP>
-hipError_t launch_heavy_t(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
-                          int32_t bw_max, int32_t qcap, int64_t n_max, int32_t* spill, hipStream_t s) {
-  const size_t lds = heavy_lds_bytes(bw_max, qcap);
-  switch (threads) {
-    case 1024:
-      hipLaunchKernelGGL((k_heavy<1024, kP>), dim3(grid), dim3(1024), lds, s, mats, ws, heavy_items, bw_max, qcap,
-                         n_max, spill);
-      break;
-    case 512:
-      hipLaunchKernelGGL((k_heavy<512, kP>), dim3(grid), dim3(512), lds, s, mats, ws, heavy_items, bw_max, qcap,
-                         n_max, spill);
-      break;
-    case 256:
-      hipLaunchKernelGGL((k_heavy<256, kP>), dim3(grid), dim3(256), lds, s, mats, ws, heavy_items, bw_max, qcap,
-                         n_max, spill);
-      break;
-    default:
-      return hipErrorInvalidValue;
+hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap) {
+  for (bool paired : {false, true}) {
+    const void* f = heavy_fn(threads, paired, hbm_bitmap);
+    if (!f) return hipErrorInvalidValue;
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
-                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill, hipStream_t s) {
-  if (paired_any) return launch_heavy_t<true>(threads, grid, mats, ws, heavy_items, bw_max, qcap, n_max, spill, s);
-  return launch_heavy_t<false>(threads, grid, mats, ws, heavy_items, bw_max, qcap, n_max, spill, s);
+                        int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
+                        uint32_t* hbm_bitmaps, hipStream_t s) {
+  const bool hbm = hbm_bitmaps != nullptr;
+  const void* f = heavy_fn(threads, paired_any != 0, hbm);
+  if (!f) return hipErrorInvalidValue;
+  void* args[] = {&mats, &ws, &heavy_items, &bw_max, &qcap, &n_max, &spill, &hbm_bitmaps};
+  return hipLaunchKernel(f, dim3(grid), dim3(threads), args, heavy_lds_bytes(bw_max, qcap, hbm), s);
 }
 
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
@@ -1613,22 +1638,16 @@ hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipS
   return hipGetLastError();
 }
 
-template <bool kP>
-int heavy_occupancy_t(int threads, size_t lds_bytes) {
-  int nb = 0;
-  hipError_t e = hipErrorInvalidValue;
-  switch (threads) {
-    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<1024, kP>, 1024, lds_bytes); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<512, kP>, 512, lds_bytes); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_heavy<256, kP>, 256, lds_bytes); break;
-    default: break;
+// Resident workgroups per CU of both paired variants (the grid must fit either).
+int heavy_occupancy(int threads, size_t lds_bytes, bool hbm_bitmap) {
+  int occ = 1 << 30;
+  for (bool paired : {false, true}) {
+    const void* f = heavy_fn(threads, paired, hbm_bitmap);
+    int nb = 0;
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds_bytes) != hipSuccess) return 0;
+    occ = std::min(occ, nb);
   }
-  return e == hipSuccess ? nb : 0;
-}
-
-// Resident workgroups per CU of both variants (the grid must fit either).
-int heavy_occupancy(int threads, size_t lds_bytes) {
-  return std::min(heavy_occupancy_t<false>(threads, lds_bytes), heavy_occupancy_t<true>(threads, lds_bytes));
+  return occ;
 }
 
 int light_occupancy() {

@@ -135,6 +135,10 @@ typedef struct gsofa_options {
   int32_t autotune;         /* 1: at creation, time the bitmap-path workgroup shapes and larger
                                row blocks on this very set and keep the fastest (results are
                                identical under every candidate); 0 (default): built-in rules */
+  int32_t bitmap_in_hbm;    /* 1: the bitmap path keeps its n-bit bitmaps in HBM (L2 atomics);
+                               0 (default): only where one does not fit in LDS (n above about
+                               1.2 M rows), so any n runs. The reference's labels live in global
+                               memory at every n (SOA:470-479).                                */
 } gsofa_options;
 
 typedef struct gsofa_result {

@@ -165,6 +165,57 @@ def test_erdos_renyi_members_pair_automatically(member):
         check(p, structure=True, label=f"batch member {member} unpaired", pair_directions=0)
 
 
+@pytest.mark.parametrize("case", ["grid2d_4096", "ds_11948", "random_long_rows_paired", "member_7", "grid3d_chain1"])
+def test_bitmaps_in_hbm_forced(case):
+    """The bitmap path with its bitmaps in HBM (L2 atomics, fenced barriers), forced at sizes the
+    oracle checks: plain and paired items, long rows, chaining on and off, with the structure."""
+    if case == "grid2d_4096":
+        check(g.generate_workload(g.WL_GRID2D_4096), label="grid2d_4096 hbm", bitmap_in_hbm=1)
+    elif case == "ds_11948":
+        check(load_ds(), S=128, label="ds_11948 hbm", bitmap_in_hbm=1)
+    elif case == "random_long_rows_paired":
+        rng = np.random.default_rng(23)
+        n = 3000
+        D = rng.random((n, n)) < 0.002
+        np.fill_diagonal(D, rng.random(n) < 0.9)
+        D[rng.integers(0, n, 10)] |= rng.random((10, n)) < 0.03
+        check(g.Pattern.from_dense(D), label="random long rows paired hbm", bitmap_in_hbm=1, pair_directions=1,
+              chain_block=3)
+    elif case == "member_7":
+        p = g.generate_workload(g.WL_BATCH256, 7)
+        check(p, structure=p.n < 3000, label="batch member 7 hbm", bitmap_in_hbm=1)
+    else:
+        check(g.generate_workload(g.WL_GRID2D_4096), label="grid2d_4096 hbm chain 1", bitmap_in_hbm=1,
+              chain_block=1)
+
+
+def test_large_n_takes_hbm_bitmaps():
+    """n = 1,500,160 (a bitmap of 188 KB does not fit in LDS): 1,465 copies of one 1,024-row
+    random block on the diagonal. Blocks never interact and align with the supernode chunks, so
+    every total is the block's oracle total times the copies and nzU repeats the block's."""
+    rng = np.random.default_rng(31)
+    nb, copies = 1024, 1465
+    D = rng.random((nb, nb)) < 6.0 / nb
+    np.fill_diagonal(D, True)
+    D &= ~(np.triu(rng.random((nb, nb)) < 0.1, 1))  # asymmetric: drop 10 % of the upper entries
+    blk = g.Pattern.from_dense(D)
+    o = oracle.symbfact(blk, chunk=128, structure=False)
+    deg = blk.end - blk.beg
+    beg = np.concatenate([[0], np.cumsum(np.tile(deg, copies))[:-1]]).astype(np.int32)
+    end = (beg + np.tile(deg, copies)).astype(np.int32)
+    rows_adj = np.concatenate([blk.adj[blk.beg[i]:blk.end[i]] for i in range(nb)]).astype(np.int64)
+    adj = (np.tile(rows_adj, copies) + np.repeat(np.arange(copies, dtype=np.int64) * nb, len(rows_adj))).astype(np.int32)
+    p = g.Pattern(nb * copies, beg, end, adj)
+    say(f"[large n] n={p.n} nnz={p.nnz}")
+    with g.Plan(p, chunk_size=128) as plan:
+        plan.run()
+        r = plan.result()
+        say(f"[large n] {r.time_ms:.1f} ms, heavy sources {r.heavy_sources}, stages {plan.stage_times()}")
+        assert (r.fill_count, r.nnz_F, r.edge_checks, r.reach, r.supernodes_counted) == \
+            (copies * o["fill"], copies * o["nnz_F"], copies * o["W"], copies * o["R"], copies * o["nsup"])
+        assert np.array_equal(plan.nzu(), np.tile(o["nzu"], copies))
+
+
 def test_tuning_configurations_are_identical():
     p = load_ds()
     base = None
@@ -276,6 +327,14 @@ def test_full_size_against_committed_oracle_values(name, wl):
         assert hashlib.sha256(plan.nzu().astype(np.int32).tobytes()).hexdigest() == gd["nzu_sha256_S128"]
         assert hashlib.sha256(plan.supernode_flags().tobytes()).hexdigest() == gd["sup_flag_sha256_S128"]
         say(f"[full size] {name}: {r.time_ms:.2f} ms, heavy sources {r.heavy_sources}, stages {plan.stage_times()}")
+    if name == "grid2d_659033":  # the same with the bitmaps in HBM
+        with g.Plan(p, chunk_size=128, bitmap_in_hbm=1) as plan:
+            plan.run()
+            r = plan.result()
+            assert (r.fill_count, r.nnz_F, r.edge_checks, r.reach, r.supernodes_counted) == \
+                (o["fill"], o["nnz_F"], o["W"], o["R"], o["nsup"])
+            assert hashlib.sha256(plan.nzu().astype(np.int32).tobytes()).hexdigest() == gd["nzu_sha256_S128"]
+            say(f"[full size] {name} hbm bitmaps: {r.time_ms:.2f} ms, stages {plan.stage_times()}")
 
 
 def test_full_size_structure_properties():
