@@ -11,14 +11,16 @@
 //     reached set (or to j) is in L column j, since the reversed path has all
 //     intermediates below j = min(m, j).
 // A reachability traversal needs one bit of state per vertex instead of a 32-bit minimax
-// label, so a whole traversal of a 1.2M-vertex pattern fits in one CU's LDS.
+// label, so a whole traversal of a 1.2M-vertex pattern fits in one CU's LDS (larger patterns
+// keep the bitmap in HBM).
 //
 // Kernels, in launch order for one matrix or one batch:
 //   k_fill_u64 / k_fill_i32     clear accumulators and counters (graph-safe, no memset nodes)
 //   k_ell_build                 pack rows into 64-byte ELL-16 slots
 //   k_col_hist, scan, k_transpose_scatter, k_ell_build   build A^T in the same layout
-//   k_light                     one wave per source, LDS hash set; escalates big sources
-//   k_heavy                     one workgroup per escalated source, n-bit LDS bitmap
+//   k_light                     one wave per block of rows, LDS hash set; escalates big sources
+//   k_heavy_hist, scan, k_heavy_scatter   the workgroup path's queue, costliest blocks first
+//   k_heavy                     one workgroup per block of escalated rows, n-bit bitmap
 //   k_supernodes                the reference's supernode rule, one thread per row
 #include <hip/hip_runtime.h>
 #include <stdint.h>
