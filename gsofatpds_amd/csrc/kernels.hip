@@ -524,16 +524,21 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             const int vi = head + (lane >> 4);
             const int32_t v = vi < qn ? Q[vi] : -1;
             const int32_t m = v >= 0 ? gload(M.ell[dir] + (int64_t)v * kEll + (lane & (kEll - 1))) : kPad;
+            // a queued vertex is in R_src: if its own row escalated, X_src contains its X
+            const bool vesc = v >= 0 && (lane & (kEll - 1)) == 0 && gload(M.esc[dir] + v) != 0;
             const int qend = qn;
             narrow(v, m, false);
+            if (__ballot(vesc) != 0) esc = true;
             head = qend;
             continue;
           }
           int batch_end = 0;
+          bool vesc_any = false;  // a vertex of this batch (in R_src) escalated: so does src
           {
             batch_end = min(head + 64, qn);
             const int32_t v = (head + lane < batch_end) ? Q[head + lane] : -1;
             load_row(M, dir, v, rc);
+            vesc_any = v >= 0 && gload(M.esc[dir] + v) != 0;  // tested after the claims
           }
           uint32_t deg = 0;
           // kLightStep slots at a time: their home-slot CAS operations are in flight together;
@@ -594,6 +599,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             hcount += (int)wave_sum_u32(nnew);
             if (hcount > limit || qn >= kLightQ) esc = true;
           }
+          if (__ballot(vesc_any) != 0) esc = true;
           auto visit = [&](int32_t m) {
             const bool active = m >= 0;
             deg += active;
