@@ -224,8 +224,9 @@ def main():
     gteps = W / (ms_per_step / 1e3) / 1e9
     fill_rate = r.fill_count / (ms_per_step / 1e3)
     # Roofline of the dominant kernel, priced on the work that kernel executed (both directions):
-    # 8 B per neighbour inspection (index + label of the reference's loop, SOA:952, 959) + 12 B
-    # per expanded row (row extent + label write, SOA:923, 948) + 4 B per F entry written. (The
+    # 8 B per neighbour inspection (index + label of the reference's loop,
+    # gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:952, 959) + 12 B per expanded
+    # row (row extent + label write, same file :923, 948) + 4 B per F entry written. (The
     # schedule-independent W would overstate it: chaining and the second direction change how
     # many inspections one step really does.)
     kern = {"k_heavy": (xw["heavy_edges"], xw["heavy_rows"], xw["heavy_out"], stages["heavy_ms"]),
