@@ -187,14 +187,24 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     return fail(GSOFA_ERR_CAPACITY, "bitmaps in HBM run 1024-thread workgroups only");
   const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : 1024 / heavy_threads_;
   const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
+  // U lists (explicit structure only): their first 4,096 (or 1,024) entries per direction in LDS
+  // when the frontier queues keep at least 2,048 entries per level; the rest stays in HBM.
+  const int64_t rem = budget > bitmap_bytes + reserve ? (int64_t)(budget - bitmap_bytes - reserve) : 0;
+  const int64_t udirs = paired_any_ ? 2 : 1;
+  ucap_ = 0;
+  if (opt.structure) {
+    const int64_t qmin = opt.qcap ? opt.qcap : 2048;
+    for (int32_t c : {4096, 1024})
+      if (!ucap_ && rem >= 8 * qmin + 4 * udirs * c) ucap_ = c;
+  }
   if (opt.qcap) {
     qcap_ = opt.qcap;
   } else {
-    int64_t q = budget > bitmap_bytes + reserve ? (int64_t)(budget - bitmap_bytes - reserve) / 8 : 256;
+    int64_t q = rem > 4 * udirs * ucap_ ? (rem - 4 * udirs * ucap_) / 8 : 256;
     q = std::max<int64_t>(256, std::min<int64_t>(16384, q));
     qcap_ = (int32_t)(q & ~int64_t(63));
   }
-  const size_t lds = heavy_lds_bytes(bw_max_, qcap_, hbm_bitmap_);
+  const size_t lds = heavy_lds_bytes(bw_max_, qcap_, hbm_bitmap_, ucap_, paired_any_ != 0);
   if (lds + reserve > info.lds_per_block) return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
   GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds, hbm_bitmap_), "hipFuncSetAttribute(k_heavy)");
   const int occ = heavy_occupancy(heavy_threads_, lds, hbm_bitmap_);
@@ -393,7 +403,7 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
             "workgroup-path queue");
   if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
   GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, paired_any_,
-                         spill_, hbm_bm_, s),
+                         spill_, hbm_bm_, ucap_, s),
             "k_heavy");
   if (ev) GSOFA_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
   GSOFA_HIP(launch_supernodes(mats_d_, nmats_, blk_base_d_, nblocks_sup_, s), "k_supernodes");

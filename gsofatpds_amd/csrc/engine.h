@@ -134,6 +134,7 @@ class DeviceSet {
   int32_t* bucket_ = nullptr;         // nb_max: escalated blocks per block index
   bool hbm_bitmap_ = false;           // bitmap-path bitmaps in HBM (hbm_bm_), not LDS
   uint32_t* hbm_bm_ = nullptr;        // heavy_grid * bw_max words
+  int32_t ucap_ = 0;                  // bitmap-path U-list entries per direction kept in LDS
   int64_t *list_ptr_u_ = nullptr, *list_ptr_l_ = nullptr;  // nrows + nmats entries
   int32_t *pool_u_ = nullptr, *pool_l_ = nullptr;
   std::vector<int64_t> ptr_base_h_;                        // matrix b's offsets start at row_base + b

@@ -98,8 +98,9 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
                               int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s);
 // LDS of k_heavy outside its dynamic part (static shared state, 2.6 KB) plus margin.
 constexpr size_t kHeavyLdsReserve = 3072;
-// Dynamic LDS of k_heavy: the bitmap (unless it lives in HBM) and two frontier queue levels.
-size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap);
+// Dynamic LDS of k_heavy: the bitmap (unless it lives in HBM), two frontier queue levels, and
+// the first ucap entries of the staged U list of each direction (two with paired items).
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap, int32_t ucap, bool paired_any);
 hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap);
 // Saved bitmaps per direction in a workgroup region when some matrix runs paired items: a ring
 // of the last rows saved (chain bases for later rows, early tests for the other direction).
@@ -114,7 +115,7 @@ constexpr int64_t heavy_region_ints(int64_t n_max, bool paired_any) {
 // (1024-thread workgroups only).
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
-                        uint32_t* hbm_bitmaps, hipStream_t s);
+                        uint32_t* hbm_bitmaps, int32_t ucap, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
 constexpr int kLightWaves = 4;  // wavefronts per wave-path workgroup

@@ -743,8 +743,9 @@ __global__ void k_heavy_scatter(const MatDev* __restrict__ mats, int64_t nblk_to
 // ------------------------------------------------------------------------------------------
 // k_heavy: one workgroup per escalated source, n-bit bitmap in LDS, level-synchronous
 // ------------------------------------------------------------------------------------------
-// Dynamic LDS: bitmap[bw_max] | q0[qcap] | q1[qcap]. Queue indices >= qcap spill to the
-// workgroup's global region: spill0 / spill1 (n_max entries each), ustage (n_max entries).
+// Dynamic LDS: bitmap[bw_max] | q0[qcap] | q1[qcap] | U list[ucap] per direction. Queue
+// indices >= qcap and U-list indices >= ucap continue in the workgroup's global region:
+// spill0 / spill1 (n_max entries each), ustage (n_max entries per direction).
 __device__ __forceinline__ int32_t qget(const int32_t* l, const int32_t* g, int32_t qcap, int32_t e) {
   return e < qcap ? l[e] : gload(g + (e - qcap));
 }
@@ -821,7 +822,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                     const uint64_t* __restrict__ heavy_items,
                                                     int32_t bw_max, int32_t qcap, int64_t n_max,
                                                     int32_t* __restrict__ spill_all,
-                                                    uint32_t* __restrict__ hbm_bitmaps) {
+                                                    uint32_t* __restrict__ hbm_bitmaps, int32_t ucap) {
   static_assert(kEll == 16, "the window scan maps one row to 16 lanes");
   extern __shared__ uint32_t dyn[];
   constexpr int kWaves = kThreads / 64;
@@ -845,6 +846,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   // the n-bit bitmap: LDS, or this workgroup's bw_max words of hbm_bitmaps
   uint32_t* const bm = kHbmBitmap ? hbm_bitmaps + (int64_t)blockIdx.x * bw_max : dyn;
   int32_t* const ql0 = reinterpret_cast<int32_t*>(kHbmBitmap ? dyn : dyn + bw_max);
+  // the staged U list of each direction: its first ucap entries in LDS, the rest in ustage
+  int32_t* const ul0 = ql0 + 2 * qcap;
+  int32_t* const ul1 = kPairedAny ? ul0 + ucap : ul0;
   // HBM bitmap: its claims are L2 atomics, so its reads bypass L1 (agent-scope loads, never a
   // stale line), and each wave's plain stores (clears, copies) are acknowledged by L2 before a
   // barrier lets another wave's atomics at the same words.
@@ -955,6 +959,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           if (s_esc[d][li] == 0) continue;  // the wave path finished this row
           const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
           int32_t* const ustage = d ? ustage1 : ustage0;
+          int32_t* const ul = d ? ul1 : ul0;
+          auto uget = [&](uint32_t i) -> int32_t {
+            return (int32_t)i < ucap ? ul[i] : gload(ustage + ((int32_t)i - ucap));
+          };
+          auto uput = [&](uint32_t i, int32_t v) {
+            if ((int32_t)i < ucap) ul[i] = v;
+            else gstore(ustage + ((int32_t)i - ucap), v);
+          };
           const int32_t prev_d = d ? prev1 : prev0;
           // t1: the last row traversed here in this direction, even with wave-path rows in
           // between (plain items: s-1 only)
@@ -1062,8 +1074,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               // thread that finds it moves the last entry into its place.
               const uint32_t len = s_facts[d][fi].ul;
               for (uint32_t i = tid; i < len; i += kThreads)
-                if (gload(ustage + i) == src) {
-                  gstore(ustage + i, gload(ustage + len - 1));
+                if (uget(i) == src) {
+                  uput(i, uget(len - 1));
                   s_drop = 1;
                 }
             } else if (want_list) {
@@ -1071,7 +1083,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               const uint32_t len = s_facts[d][fi].ul;
               for (uint32_t base = 0; base < len; base += kThreads) {
                 const uint32_t i = base + tid;
-                const int32_t m = i < len ? gload(ustage + i) : -1;
+                const int32_t m = i < len ? uget(i) : -1;
                 const bool keep = m > src;
                 const uint64_t bal = __ballot(keep);
                 if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
@@ -1081,7 +1093,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   if (w < wave) before += s_scan[w];
                   all += s_scan[w];
                 }
-                if (keep) gstore(ustage + kept + before + rank_below(bal), m);
+                if (keep) uput(kept + before + rank_below(bal), m);
                 kept += all;
                 bsync();
               }
@@ -1227,7 +1239,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   uint32_t pos = ubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
 #pragma unroll
                   for (int j = 0; j < kEll; ++j)
-                    if ((mask_u >> j) & 1u) gstore(ustage + pos++, rc.e[j]);
+                    if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
                 }
               }
               // the CSR tail of a long row, one entry per lane per step
@@ -1263,7 +1275,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     uint32_t basepos = 0;
                     if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
                     basepos = ubase + __shfl(basepos, leader, 64);
-                    if (is_u) gstore(ustage + basepos + rank_below(mu), m);
+                    if (is_u) uput(basepos + rank_below(mu), m);
                   }
                 }
               };
@@ -1326,7 +1338,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     while (nb) {
                       const int b = __ffs(nb) - 1;
                       nb &= nb - 1;
-                      gstore(ustage + pos++, w * 32 + b);
+                      uput(pos++, w * 32 + b);
                     }
                   }
                 }
@@ -1428,14 +1440,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             int32_t* const dst = M.pool[d] + s_lp[d][li];
             uint32_t i = tid;
             for (; i + 3 * kThreads < U; i += 4 * kThreads) {
-              const int32_t a = gload(ustage + i), b = gload(ustage + i + kThreads);
-              const int32_t c = gload(ustage + i + 2 * kThreads), e = gload(ustage + i + 3 * kThreads);
+              const int32_t a = uget(i), b = uget(i + kThreads);
+              const int32_t c = uget(i + 2 * kThreads), e = uget(i + 3 * kThreads);
               gstore(dst + i, a);
               gstore(dst + i + kThreads, b);
               gstore(dst + i + 2 * kThreads, c);
               gstore(dst + i + 3 * kThreads, e);
             }
-            for (; i < U; i += kThreads) gstore(dst + i, gload(ustage + i));
+            for (; i < U; i += kThreads) gstore(dst + i, uget(i));
           }
           if (save) {
             uint32_t* const gout = gslot(d, wslot);
@@ -1578,8 +1590,8 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
   return hipGetLastError();
 }
 
-size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap) {
-  return (hbm_bitmap ? 0 : (size_t)bw_max * 4) + (size_t)qcap * 8;
+size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap, int32_t ucap, bool paired_any) {
+  return (hbm_bitmap ? 0 : (size_t)bw_max * 4) + (size_t)qcap * 8 + (size_t)ucap * 4 * (paired_any ? 2 : 1);
 }
 
 // The instantiation for (threads, paired, bitmap in HBM); the HBM bitmap runs 1024-thread
@@ -1615,12 +1627,13 @@ hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap) {
 
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
-                        uint32_t* hbm_bitmaps, hipStream_t s) {
+                        uint32_t* hbm_bitmaps, int32_t ucap, hipStream_t s) {
   const bool hbm = hbm_bitmaps != nullptr;
   const void* f = heavy_fn(threads, paired_any != 0, hbm);
   if (!f) return hipErrorInvalidValue;
-  void* args[] = {&mats, &ws, &heavy_items, &bw_max, &qcap, &n_max, &spill, &hbm_bitmaps};
-  return hipLaunchKernel(f, dim3(grid), dim3(threads), args, heavy_lds_bytes(bw_max, qcap, hbm), s);
+  void* args[] = {&mats, &ws, &heavy_items, &bw_max, &qcap, &n_max, &spill, &hbm_bitmaps, &ucap};
+  return hipLaunchKernel(f, dim3(grid), dim3(threads), args,
+                         heavy_lds_bytes(bw_max, qcap, hbm, ucap, paired_any != 0), s);
 }
 
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
