@@ -263,8 +263,9 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
   GSOFA_TRY(alloc(ws_, 1, "work state"));
   GSOFA_TRY(alloc(heavy_items_, 2 * nblk_total_, "escalation list"));
-  // per resident workgroup: two frontier spill queues, the U list and the saved bitmap of the
-  // last row (with paired items: a U list per direction, a bitmap per direction and row parity)
+  // per resident workgroup: two frontier spill queues, the U list (beyond its LDS part) and the
+  // saved bitmap of the last row (with paired items: a U list per direction and a ring of the
+  // last saved rows' bitmaps per direction)
   GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (size_t)heavy_region_ints(n_max_, paired_any_ != 0),
                   "heavy spill regions"));
   if (hbm_bitmap_) GSOFA_TRY(alloc(hbm_bm_, (size_t)heavy_grid_ * (size_t)bw_max_, "heavy bitmaps"));
