@@ -32,6 +32,29 @@ def test_header_compiles_as_c():
     assert r.returncode == 0, r.stderr
 
 
+def test_binding_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of gsofa_options, gsofa_result and gsofa_server_options have the C
+    sizes and field offsets, and the defaults are the documented ones."""
+    import ctypes
+    prog = tmp_path / "sizes.c"
+    fields = [f for f, _ in g.Options._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gsofa.h"', "int main(void) {",
+             '  printf("%zu %zu %zu\\n", sizeof(gsofa_options), sizeof(gsofa_result), sizeof(gsofa_server_options));']
+    lines += [f'  printf("%zu\\n", offsetof(gsofa_options, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    prog.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "sizes"
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    sizes, offsets = [int(x) for x in out[:3]], [int(x) for x in out[3:]]
+    assert sizes == [ctypes.sizeof(g.Options), ctypes.sizeof(g.Result), ctypes.sizeof(g.ServerOptions)]
+    assert offsets == [getattr(g.Options, f).offset for f in fields]
+    o = g.default_options()
+    assert (o.chunk_size, o.pair_directions, o.autotune, o.bitmap_in_hbm, o.structure) == (128, -1, 0, 0, 0)
+
+
 def test_workload_sizes():
     p1 = g.generate_workload(g.WL_GRID2D_4096)
     assert p1.n == 4096 and 16000 < p1.nnz < 18000
