@@ -155,6 +155,38 @@ def split_structure(n, F_rowptr, F_cols):
     return u_ptr, u_idx, l_ptr, l_idx
 
 
+def split_structure_fast(n, F_rowptr, F_cols):
+    """split_structure for large F (vectorised): strict U by rows, strict L by columns, sorted."""
+    ptr = np.asarray(F_rowptr, np.int64)
+    cols = np.asarray(F_cols, np.int32)
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(ptr))
+    up = cols > rows
+    u_idx = cols[up]
+    u_ptr = np.zeros(n + 1, np.int64)
+    u_ptr[1:] = np.cumsum(np.bincount(rows[up], minlength=n))
+    lo = cols < rows
+    lc, lr = cols[lo], rows[lo]
+    del rows, up, lo
+    order = np.lexsort((lr, lc))  # by column, then row
+    l_idx = lr[order]
+    l_ptr = np.zeros(n + 1, np.int64)
+    l_ptr[1:] = np.cumsum(np.bincount(lc, minlength=n))
+    return u_ptr, u_idx, l_ptr, l_idx
+
+
+def structure_digests(u_ptr, u_idx, l_ptr, l_idx):
+    """SHA-256 of strict U by rows (int64 offsets, then int32 columns) and of strict L by columns
+    (int64 offsets, then int32 rows): the form gsofa_structure_get returns."""
+    import hashlib
+    hu = hashlib.sha256()
+    hu.update(np.ascontiguousarray(u_ptr, np.int64).tobytes())
+    hu.update(np.ascontiguousarray(u_idx, np.int32).tobytes())
+    hl = hashlib.sha256()
+    hl.update(np.ascontiguousarray(l_ptr, np.int64).tobytes())
+    hl.update(np.ascontiguousarray(l_idx, np.int32).tobytes())
+    return hu.hexdigest(), hl.hexdigest()
+
+
 # ---- independent checkers (from the mathematics, not from the reference) --------------------
 
 def dense_elimination(dense: np.ndarray) -> np.ndarray:

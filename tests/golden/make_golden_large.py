@@ -1,4 +1,5 @@
-"""Oracle values for the 110,592-row and 659,033-row workloads (tests/golden/golden_large.json).
+"""Oracle values for the 110,592-row and 659,033-row workloads (tests/golden/golden_large.json):
+totals, digests of nzU and the supernode flags, and digests of the explicit L/U structure.
 
 Too slow for a test (the label-correcting restatement takes minutes at these sizes), so they are
 computed once here and committed; the GPU tests compare the engine against them at full size.
@@ -32,6 +33,13 @@ def main(names):
                                                            "edge_checks")}
             entry[f"nzu_sha256_S{S}"] = hashlib.sha256(o["nzu"].astype(np.int32).tobytes()).hexdigest()
             entry[f"sup_flag_sha256_S{S}"] = hashlib.sha256(o["sup_flag"].astype(np.uint8).tobytes()).hexdigest()
+        # the explicit structure: strict U by rows and strict L by columns, as digests
+        o = oracle.symbfact(p, chunk=128, structure=True, stats=False)
+        parts = oracle.split_structure_fast(p.n, o["F_rowptr"], o["F_cols"])
+        del o
+        entry["U_rows_sha256"], entry["L_cols_sha256"] = oracle.structure_digests(*parts)
+        entry["nnz_U"], entry["nnz_L"] = int(parts[0][-1]), int(parts[2][-1])
+        del parts
         entry["oracle_seconds"] = time.time() - t0
         data[name] = entry
         with open(OUT, "w") as f:

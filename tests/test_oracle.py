@@ -211,6 +211,161 @@ def test_supernodes_lower_bidiagonal_plus_upper_chain():
     assert o["nsup"] == n - 1
 
 
+# ---- supernode rule: the walk after a failed pass-through ----------------------------------------
+# In phase 2 (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:2028-2049) a walk from
+# start s tests every following pass-through row r against the same s: (r, s) in F extends the
+# run (:2034-2038), otherwise one more supernode is counted (:2047) and the walk goes on, still
+# comparing against s (no start or end is recorded, :2048-2049). The cases below separate that
+# from the plausible misreading "the failed row starts a new run".
+
+def rows_to_dense(n, rows):
+    D = np.zeros((n, n), bool)
+    for i, r in enumerate(rows):
+        D[i, r] = True
+    return D
+
+
+def test_supernode_failed_pass_through_keeps_the_start():
+    """n = 3, rows {0,1,2}, {1,2}, {1,2}. nzU = 2, 1, 0, so rows 1 and 2 pass the first filter.
+    (1,0) is not in F (row 1 has no lower entry) and (2,0) is not in F (row 2's only lower entry
+    is 1, and no vertex below 0 exists), so both fail against start 0: 3 supernodes.
+    Restarting at row 1 would extend row 2, since (2,1) is in A: 2 supernodes."""
+    o = oracle.symbfact(dense_to_pattern(rows_to_dense(3, [[0, 1, 2], [1, 2], [1, 2]])), chunk=128)
+    assert list(o["nzu"]) == [2, 1, 0]
+    assert o["nsup"] == 3
+    assert list(o["sup_flag"]) == [1, 1, 1]
+
+
+def test_supernode_row_after_a_failure_extends_against_the_start():
+    """n = 3, rows {0,1,2}, {1,2}, {0,2}: row 1 fails against start 0, row 2 holds (2,0) in A and
+    extends the run of row 0: 2 supernodes, flags 1, 1, 0."""
+    o = oracle.symbfact(dense_to_pattern(rows_to_dense(3, [[0, 1, 2], [1, 2], [0, 2]])), chunk=128)
+    assert o["nsup"] == 2
+    assert list(o["sup_flag"]) == [1, 1, 0]
+
+
+def test_supernode_walk_dense_upper_hand_worked():
+    """n = 6, the whole upper triangle present, lower entries: row 2 {1}, row 3 {0}, row 5 {2}.
+    nzU[r] = 5 - r, so every row after 0 is a pass-through. With U dense, (r, s) in F iff row r
+    has a lower entry x <= s (r -> x -> s, x < s, or x = s). Against start 0: rows 1, 2, 4 and 5
+    fail, row 3 extends: 5 supernodes, flags 1 1 1 0 1 1. (Restarting at each failed row would give
+    3: rows 2, 3 and 5 would extend runs of rows 1, 1 and 4.)"""
+    n = 6
+    D = np.triu(np.ones((n, n), bool))
+    D[2, 1] = D[3, 0] = D[5, 2] = True
+    o = oracle.symbfact(dense_to_pattern(D), chunk=128)
+    assert list(o["nzu"]) == [5, 4, 3, 2, 1, 0]
+    assert o["nsup"] == 5
+    assert list(o["sup_flag"]) == [1, 1, 1, 0, 1, 1]
+
+
+def two_phase_supernodes(F, nzu, S, restart_on_failure=False):
+    """Supernode count and increment flags, written from the reference's two phases:
+    phase 1 (...cuh:1833-1950): the 128 threads of one block take the chunk's offsets t, t+128, ...;
+    thread 0's rows are starts (:1879), another row is a start when nzU[r-1] - nzU[r] != 1
+    (:1928), else it gets a pass-through flag (:1943). Phase 2 (:1961-2111): from each start, walk
+    r = s+1, s+2, ... while r carries a flag (flags of the next chunk are not set yet, so the walk
+    ends at the chunk's end): (r, s) in F extends the run, else one more supernode.
+    restart_on_failure plants the misreading: a failed row becomes the start of the walk."""
+    n = len(nzu)
+    flags = np.zeros(n, np.uint8)
+    count = 0
+    for c0 in range(0, n, S):
+        c1 = min(c0 + S, n)
+        starts, passing = [], np.zeros(c1 - c0, bool)
+        for off in range(c1 - c0):
+            r = c0 + off
+            if off % 128 == 0 or nzu[r - 1] - nzu[r] != 1:
+                starts.append(r)
+            else:
+                passing[off] = True
+        for s in starts:
+            count += 1
+            flags[s] = 1
+            base = s
+            r = s + 1
+            while r < c1 and passing[r - c0]:
+                if not F[r, base]:
+                    count += 1
+                    flags[r] = 1
+                    if restart_on_failure:
+                        base = r
+                r += 1
+    return count, flags
+
+
+def pass_through_family(n, p_lower, seed):
+    """Whole upper triangle plus random lower entries: every row passes the first filter, and
+    (r, s) in F depends on row r's lowest entry, so both outcomes of the second filter occur."""
+    rng = np.random.default_rng(seed)
+    D = np.triu(np.ones((n, n), bool)) | np.tril(rng.random((n, n)) < p_lower, -1)
+    return D
+
+
+@pytest.mark.parametrize("S", [64, 128, 256])
+@pytest.mark.parametrize("family,n,param,seed", [("upper", 256, 0.01, 1), ("upper", 200, 0.03, 2),
+                                                  ("upper", 131, 0.005, 3), ("random", 256, 0.02, 4),
+                                                  ("random", 180, 0.05, 5), ("band", 256, 6, 6)])
+def test_supernodes_equal_two_phase_brute_force(S, family, n, param, seed):
+    """The oracle's supernodes against the two-phase restatement run on F from dense elimination
+    and nzU counted from that F. The inputs exercise failed pass-throughs, extensions after a
+    failure and, for S = 256, the forced start at offset 128; the planted restart variant must
+    disagree with the oracle wherever a failure is followed by a row of the same walk."""
+    rng = np.random.default_rng(seed)
+    if family == "upper":
+        D = pass_through_family(n, param, seed)
+    elif family == "random":
+        D = rng.random((n, n)) < param
+        np.fill_diagonal(D, True)
+    else:  # an upper band of random width per row and a sparse lower part
+        D = np.eye(n, dtype=bool) | np.tril(rng.random((n, n)) < 0.01, -1)
+        for i in range(n):
+            D[i, i + 1:i + 1 + int(rng.integers(0, param))] = True
+    E = oracle.dense_elimination(D)
+    np.fill_diagonal(E, True)
+    nzu = np.triu(E, 1).sum(axis=1)
+    count, flags = two_phase_supernodes(E, nzu, S)
+    o = oracle.symbfact(dense_to_pattern(D), chunk=S)
+    assert np.array_equal(o["nzu"], nzu)
+    assert o["nsup"] == count
+    assert np.array_equal(o["sup_flag"], flags)
+    if family == "upper":
+        # not vacuous: failures, and rows extending after a failure in the same walk
+        fails = sum(1 for r in range(1, n) if flags[r] and (r % S) % 128 != 0)
+        assert fails > 0 and fails < n - 1 - (n - 1) // 128
+        wrong, wflags = two_phase_supernodes(E, nzu, S, restart_on_failure=True)
+        assert wrong != o["nsup"] or not np.array_equal(wflags, o["sup_flag"])
+
+
+def test_planted_restart_variant_disagrees_on_the_hand_case():
+    D = rows_to_dense(3, [[0, 1, 2], [1, 2], [1, 2]])
+    E = oracle.dense_elimination(D)
+    np.fill_diagonal(E, True)
+    nzu = np.triu(E, 1).sum(axis=1)
+    assert two_phase_supernodes(E, nzu, 128)[0] == 3
+    assert two_phase_supernodes(E, nzu, 128, restart_on_failure=True)[0] == 2
+
+
+def test_supernodes_reference_dataset_two_phase():
+    """The reference's 11,948-row input at S = 128 and 1024: the oracle's count and flags equal the
+    two-phase restatement run on the oracle's own F (F itself is pinned by 980,136 above)."""
+    ds = load_ds()
+    for S in (128, 1024):
+        o = oracle.symbfact(ds, chunk=S, structure=True)
+        n = ds.n
+        ptr, cols = o["F_rowptr"], o["F_cols"]
+
+        class RowsF:  # (r, s) in F by binary search in the sorted F row r
+            def __getitem__(self, rs):
+                r, s = rs
+                row = cols[ptr[r]:ptr[r + 1]]
+                k = np.searchsorted(row, s)
+                return k < len(row) and row[k] == s
+        count, flags = two_phase_supernodes(RowsF(), o["nzu"], S)
+        assert o["nsup"] == count, S
+        assert np.array_equal(o["sup_flag"], flags)
+
+
 # ---- invariances -----------------------------------------------------------------------------
 
 def test_invariance_chunk_and_threads():
