@@ -21,6 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsofa.so")
 CLI_PATH = os.path.join(_HERE, "gsofa_cli")
+CONVERT_PATH = os.path.join(_HERE, "gsofa_convert")
 
 # status codes (include/gsofa.h)
 OK = 0
@@ -113,6 +114,23 @@ _SIGNATURES = {
                                    C.POINTER(C.c_uint64)]),
     "gsofa_executed_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "gsofa_plan_destroy": (None, [C.c_void_p]),
+    "gsofa_read_matrix_market": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "gsofa_read_permutation": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+    "gsofa_permute": (C.c_int, [C.POINTER(CSR), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_void_p)]),
+    "gsofa_supernode_partition": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]),
+    "gsofa_batch_supernode_partition": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "gsofa_supernodal_structure_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                                  C.POINTER(C.c_int64)]),
+    "gsofa_supernodal_structure_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                                 C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "gsofa_supernodal_from_structure": (C.c_int, [C.c_int32, C.POINTER(C.c_uint8), C.POINTER(C.c_int64),
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "gsofa_batch_create": (C.c_int, [C.POINTER(CSR), C.c_int32, C.POINTER(Options), C.POINTER(C.c_void_p)]),
     "gsofa_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gsofa_batch_sync": (C.c_int, [C.c_void_p]),
@@ -126,6 +144,8 @@ _SIGNATURES = {
     "gsofa_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Result)]),
     "gsofa_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(Result)]),
     "gsofa_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
+    "gsofa_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "gsofa_server_retained": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gsofa_server_request_info": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gsofa_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
@@ -244,6 +264,30 @@ def read_binary(beg_file: str, csr_file: str, index_bytes: int = 8, vert_bytes: 
 def write_binary(p: Pattern, beg_file: str, csr_file: str, index_bytes: int = 8, vert_bytes: int = 8) -> None:
     c = p.csr()
     _check(lib().gsofa_write_binary(C.byref(c), beg_file.encode(), csr_file.encode(), index_bytes, vert_bytes))
+
+
+def read_matrix_market(path: str) -> Pattern:
+    """A square coordinate Matrix Market file's pattern (symmetric kinds expanded)."""
+    h = C.c_void_p()
+    _check(lib().gsofa_read_matrix_market(path.encode(), C.byref(h)))
+    return _take_host_matrix(h)
+
+
+def read_permutation(path: str, n: int, base: int = 0) -> np.ndarray:
+    out = np.zeros(n, np.int32)
+    _check(lib().gsofa_read_permutation(path.encode(), n, base, _i32p(out)))
+    return out
+
+
+def permute(p: Pattern, perm_r=None, perm_c=None) -> Pattern:
+    """SuperLU_DIST's reordered matrix Pc * Pr * A * Pc^T: B(perm_c[perm_r[i]], perm_c[j]) = A(i, j)."""
+    c = p.csr()
+    pr = np.ascontiguousarray(perm_r, np.int32) if perm_r is not None else None
+    pc = np.ascontiguousarray(perm_c, np.int32) if perm_c is not None else None
+    h = C.c_void_p()
+    _check(lib().gsofa_permute(C.byref(c), _i32p(pr) if pr is not None else None,
+                               _i32p(pc) if pc is not None else None, C.byref(h)))
+    return _take_host_matrix(h)
 
 
 def symbfact_min_id(argv: Sequence[str], rank: int = 0):
@@ -408,6 +452,52 @@ class Plan:
                                          lp.ctypes.data_as(C.POINTER(C.c_int64)), _i32p(li)))
         return up, ui[:nu.value], lp, li[:nl.value]
 
+    def supernode_partition(self):
+        """(xsup, supno): supernode k is rows [xsup[k], xsup[k+1]); supno[i] is row i's supernode."""
+        ns = C.c_int64()
+        _check(lib().gsofa_supernode_partition(self._h, C.byref(ns), None, None))
+        xsup = np.zeros(ns.value + 1, np.int32)
+        supno = np.zeros(self.pattern.n, np.int32)
+        _check(lib().gsofa_supernode_partition(self._h, C.byref(ns), _i32p(xsup), _i32p(supno)))
+        return xsup, supno
+
+    def supernodal_structure(self):
+        """(xsup, xlsub, lsub, xusub, usub): per-supernode L panel rows and U panel columns."""
+        ns, nl, nu = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().gsofa_supernodal_structure_size(self._h, C.byref(ns), C.byref(nl), C.byref(nu)))
+        xsup = np.zeros(ns.value + 1, np.int32)
+        xl = np.zeros(ns.value + 1, np.int64)
+        xu = np.zeros(ns.value + 1, np.int64)
+        ls = np.zeros(max(nl.value, 1), np.int32)
+        us = np.zeros(max(nu.value, 1), np.int32)
+        _check(lib().gsofa_supernodal_structure_get(self._h, _i32p(xsup), xl.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                    _i32p(ls), xu.ctypes.data_as(C.POINTER(C.c_int64)), _i32p(us)))
+        return xsup, xl, ls[:nl.value], xu, us[:nu.value]
+
+
+def supernodal_from_structure(flags, u_rowptr, u_col, l_colptr, l_row):
+    """``gsofa_supernodal_from_structure`` on host arrays (no device): (xsup, xlsub, lsub, xusub, usub)."""
+    n = len(flags)
+    fl = np.ascontiguousarray(flags, np.uint8)
+    up = np.ascontiguousarray(u_rowptr, np.int64)
+    lp = np.ascontiguousarray(l_colptr, np.int64)
+    ui = np.ascontiguousarray(u_col, np.int32) if len(u_col) else np.zeros(1, np.int32)
+    li = np.ascontiguousarray(l_row, np.int32) if len(l_row) else np.zeros(1, np.int32)
+    p64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    u8 = fl.ctypes.data_as(C.POINTER(C.c_uint8))
+    ns, nl, nu = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(lib().gsofa_supernodal_from_structure(n, u8, p64(up), _i32p(ui), p64(lp), _i32p(li), C.byref(ns),
+                                                 C.byref(nl), C.byref(nu), None, None, None, None, None))
+    xsup = np.zeros(ns.value + 1, np.int32)
+    xl = np.zeros(ns.value + 1, np.int64)
+    xu = np.zeros(ns.value + 1, np.int64)
+    ls = np.zeros(max(nl.value, 1), np.int32)
+    us = np.zeros(max(nu.value, 1), np.int32)
+    _check(lib().gsofa_supernodal_from_structure(n, u8, p64(up), _i32p(ui), p64(lp), _i32p(li), C.byref(ns),
+                                                 C.byref(nl), C.byref(nu), _i32p(xsup), p64(xl), _i32p(ls), p64(xu),
+                                                 _i32p(us)))
+    return xsup, xl, ls[:nl.value], xu, us[:nu.value]
+
 
 class Batch:
     """Many matrices in one captured graph: ``gsofa_batch_*``."""
@@ -450,6 +540,15 @@ class Batch:
         out = np.zeros(self.patterns[i].n, np.int32)
         _check(lib().gsofa_batch_nzu_get(self._h, i, _i32p(out)))
         return out
+
+    def supernode_partition(self, i: int):
+        """(xsup, supno) of member i, as Plan.supernode_partition."""
+        ns = C.c_int64()
+        _check(lib().gsofa_batch_supernode_partition(self._h, i, C.byref(ns), None, None))
+        xsup = np.zeros(ns.value + 1, np.int32)
+        supno = np.zeros(self.patterns[i].n, np.int32)
+        _check(lib().gsofa_batch_supernode_partition(self._h, i, C.byref(ns), _i32p(xsup), _i32p(supno)))
+        return xsup, supno
 
     def supernode_flags(self, i: int) -> np.ndarray:
         out = np.zeros(self.patterns[i].n, np.uint8)
@@ -504,6 +603,14 @@ class Server:
         dz, ds = C.c_uint64(), C.c_uint64()
         _check(lib().gsofa_server_request_info(self._h, rid, C.byref(lat), C.byref(q), C.byref(dz), C.byref(ds)))
         return {"latency_ms": lat.value, "queue_ms": q.value, "nzu_digest": dz.value, "supernode_digest": ds.value}
+
+    def release(self, rid: int) -> None:
+        _check(lib().gsofa_server_release(self._h, rid))
+
+    def retained(self) -> dict:
+        n, b = C.c_int64(), C.c_int64()
+        _check(lib().gsofa_server_retained(self._h, C.byref(n), C.byref(b)))
+        return {"requests": n.value, "host_bytes": b.value}
 
     def stats(self) -> dict:
         c, pn, iu, b = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()

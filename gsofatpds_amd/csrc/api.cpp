@@ -171,6 +171,12 @@ struct gsofa_plan {
   }
   std::vector<MatResult> results;
   bool have_result = false;
+  bool ran = false;  // a run was enqueued: results and timings exist only after one
+  // host copy of the last run's structure and flags for the supernodal export (until the next run)
+  bool sn_cached = false;
+  std::vector<int64_t> sn_up, sn_lp;
+  std::vector<int32_t> sn_ui, sn_li;
+  std::vector<uint8_t> sn_flags;
 };
 
 struct gsofa_batch {
@@ -182,6 +188,7 @@ struct gsofa_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<MatResult> results;
   bool have_result = false;
+  bool ran = false;
   ~gsofa_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -318,11 +325,14 @@ int gsofa_run(gsofa_plan* p, void* hip_stream) {
   p->timing.stream = s;
   p->timing.pending = true;
   p->have_result = false;
+  p->ran = true;
+  p->sn_cached = false;
   return GSOFA_OK;
 }
 
 int gsofa_sync(gsofa_plan* p) {
   if (!p) return fail(GSOFA_ERR_INVALID_ARG, "plan is null");
+  if (!p->ran) return fail(GSOFA_ERR_STATE, "no run was enqueued on this plan (call gsofa_run first)");
   int rc = p->timing.finish();
   if (rc != GSOFA_OK) return rc;
   rc = p->set.read_results(p->results);
@@ -431,6 +441,73 @@ int gsofa_executed_work(gsofa_plan* p, uint64_t* work6) {
   return GSOFA_OK;
 }
 
+// ---- supernodal export -------------------------------------------------------------------
+
+static int plan_flags(gsofa_plan* p, std::vector<uint8_t>& flags) {
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  flags.resize((size_t)p->prep.n);
+  return p->set.read_counts(0, nullptr, nullptr, flags.data(), nullptr);
+}
+
+int gsofa_supernode_partition(gsofa_plan* p, int64_t* nsuper, int32_t* xsup, int32_t* supno) {
+  if (!p || !nsuper) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  std::vector<uint8_t> flags;
+  int rc = plan_flags(p, flags);
+  if (rc != GSOFA_OK) return rc;
+  *nsuper = supernode_partition(p->prep.n, flags.data(), xsup, supno);
+  return GSOFA_OK;
+}
+
+int gsofa_batch_supernode_partition(gsofa_batch* b, int32_t index, int64_t* nsuper, int32_t* xsup, int32_t* supno) {
+  if (!b || !nsuper) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (index < 0 || index >= b->set.nmats()) return fail(GSOFA_ERR_INVALID_ARG, "batch index out of range");
+  if (!b->have_result) {
+    int rc = gsofa_batch_sync(b);
+    if (rc != GSOFA_OK) return rc;
+  }
+  std::vector<uint8_t> flags((size_t)b->preps[index].n);
+  int rc = b->set.read_counts(index, nullptr, nullptr, flags.data(), nullptr);
+  if (rc != GSOFA_OK) return rc;
+  *nsuper = supernode_partition(b->preps[index].n, flags.data(), xsup, supno);
+  return GSOFA_OK;
+}
+
+static int plan_structure_cache(gsofa_plan* p) {
+  if (p->sn_cached) return GSOFA_OK;
+  if (!p->set.structure_ready()) return fail(GSOFA_ERR_STATE, "structure not produced: enable it and run first");
+  int rc = plan_flags(p, p->sn_flags);
+  if (rc != GSOFA_OK) return rc;
+  rc = p->set.read_structure(0, p->sn_up, p->sn_ui, p->sn_lp, p->sn_li);
+  if (rc != GSOFA_OK) return rc;
+  p->sn_cached = true;
+  return GSOFA_OK;
+}
+
+int gsofa_supernodal_structure_size(gsofa_plan* p, int64_t* nsuper, int64_t* nnz_lsub, int64_t* nnz_usub) {
+  if (!p || !nsuper || !nnz_lsub || !nnz_usub) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  int rc = plan_structure_cache(p);
+  if (rc != GSOFA_OK) return rc;
+  return supernodal_panels(p->prep.n, p->sn_flags.data(), p->sn_up.data(), p->sn_ui.data(), p->sn_lp.data(),
+                           p->sn_li.data(), nsuper, nnz_lsub, nnz_usub, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int gsofa_supernodal_structure_get(gsofa_plan* p, int32_t* xsup, int64_t* xlsub, int32_t* lsub, int64_t* xusub,
+                                   int32_t* usub) {
+  if (!p || !xsup || !xlsub || !xusub) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  int rc = plan_structure_cache(p);
+  if (rc != GSOFA_OK) return rc;
+  int64_t ns = 0, nl = 0, nu = 0;
+  rc = supernodal_panels(p->prep.n, p->sn_flags.data(), p->sn_up.data(), p->sn_ui.data(), p->sn_lp.data(),
+                         p->sn_li.data(), &ns, &nl, &nu, nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (rc != GSOFA_OK) return rc;
+  if ((nl > 0 && !lsub) || (nu > 0 && !usub)) return fail(GSOFA_ERR_INVALID_ARG, "lsub / usub is null");
+  return supernodal_panels(p->prep.n, p->sn_flags.data(), p->sn_up.data(), p->sn_ui.data(), p->sn_lp.data(),
+                           p->sn_li.data(), &ns, &nl, &nu, xsup, xlsub, lsub, xusub, usub);
+}
+
 void gsofa_plan_destroy(gsofa_plan* p) { delete p; }
 
 // ---- batches ---------------------------------------------------------------------------------
@@ -512,11 +589,13 @@ int gsofa_batch_run(gsofa_batch* b, void* hip_stream) {
   b->timing.stream = s;
   b->timing.pending = true;
   b->have_result = false;
+  b->ran = true;
   return GSOFA_OK;
 }
 
 int gsofa_batch_sync(gsofa_batch* b) {
   if (!b) return fail(GSOFA_ERR_INVALID_ARG, "batch is null");
+  if (!b->ran) return fail(GSOFA_ERR_STATE, "no run was enqueued on this batch (call gsofa_batch_run first)");
   int rc = b->timing.finish();
   if (rc != GSOFA_OK) return rc;
   rc = b->set.read_results(b->results);
@@ -538,6 +617,7 @@ int gsofa_batch_result_get(gsofa_batch* b, int32_t index, gsofa_result* out) {
 
 int gsofa_batch_nzu_get(gsofa_batch* b, int32_t index, int32_t* nzu) {
   if (!b || !nzu) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (index < 0 || index >= b->set.nmats()) return fail(GSOFA_ERR_INVALID_ARG, "batch index out of range");
   if (!b->have_result) {
     int rc = gsofa_batch_sync(b);
     if (rc != GSOFA_OK) return rc;
@@ -547,6 +627,7 @@ int gsofa_batch_nzu_get(gsofa_batch* b, int32_t index, int32_t* nzu) {
 
 int gsofa_batch_supernodes_get(gsofa_batch* b, int32_t index, uint8_t* flags) {
   if (!b || !flags) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (index < 0 || index >= b->set.nmats()) return fail(GSOFA_ERR_INVALID_ARG, "batch index out of range");
   if (!b->have_result) {
     int rc = gsofa_batch_sync(b);
     if (rc != GSOFA_OK) return rc;

@@ -50,6 +50,14 @@ int fail(int code, const std::string& msg);
 // Validates a caller CSR: n > 0, 0 <= beg[v] <= end[v] <= nnz, column ids in [0, n).
 int validate_csr(const gsofa_csr* a);
 
+// Supernodal export (supernodal.cpp): the partition from the supernode flags (returns the
+// number of supernodes; xsup n_super + 1 entries, supno n, either may be null) and the panels
+// of every supernode from the strict structure (sizes only when every output array is null).
+int64_t supernode_partition(int32_t n, const uint8_t* flags, int32_t* xsup, int32_t* supno);
+int supernodal_panels(int32_t n, const uint8_t* flags, const int64_t* u_ptr, const int32_t* u_idx, const int64_t* l_ptr,
+                      const int32_t* l_idx, int64_t* nsuper, int64_t* nnz_lsub, int64_t* nnz_usub, int32_t* xsup,
+                      int64_t* xlsub, int32_t* lsub, int64_t* xusub, int32_t* usub);
+
 }  // namespace gsofa
 
 struct gsofa_host_matrix {

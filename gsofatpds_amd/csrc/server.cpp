@@ -36,7 +36,8 @@ enum class State { Pending, Running, Done, Failed, Cancelled, Rejected };
 
 struct Request {
   int64_t id = 0;
-  Prepared prep;
+  int32_t n = 0;
+  Prepared prep;  // released once the request can no longer run (finished, cancelled, rejected)
   Options opt;
   int64_t footprint = 0;
   State state = State::Pending;
@@ -64,8 +65,15 @@ bool same_options(const Options& a, const Options& b) {
   return a.chunk == b.chunk && a.n_processes == b.n_processes && a.structure == b.structure &&
          a.device == b.device && a.heavy_threads == b.heavy_threads && a.heavy_wgs_per_cu == b.heavy_wgs_per_cu &&
          a.light_wgs_per_cu == b.light_wgs_per_cu && a.qcap == b.qcap && a.chain_block == b.chain_block &&
-         a.pair_directions == b.pair_directions && a.cu_share == b.cu_share;
+         a.pair_directions == b.pair_directions && a.bitmap_in_hbm == b.bitmap_in_hbm && a.cu_share == b.cu_share;
 }
+
+// Host bytes a request keeps alive (its prepared pattern).
+int64_t prep_bytes(const Prepared& p) {
+  return (int64_t)(p.rowptr.capacity() + p.adj.capacity() + p.dup_u.capacity() + p.dup_all.capacity()) * 4;
+}
+
+void release_prep(Request* r) { r->prep = Prepared(); }
 
 uint64_t fnv1a(const void* data, size_t bytes) {
   const unsigned char* p = static_cast<const unsigned char*>(data);
@@ -122,12 +130,16 @@ void gsofa_server::admit_loop() {
     if (r->cancel_requested) {
       pending.pop_front();
       r->state = State::Cancelled;
+      release_prep(r);
       cv_done.notify_all();
       continue;
     }
     // the group: the front request and the ones queued behind it, in order, while options
-    // match and the sum of their footprints fits (the group's own footprint is at most that sum)
+    // match and the group's own footprint fits. That footprint is computed for the group as a
+    // whole: one paired member sizes every workgroup region of the set for paired items, so it
+    // can exceed the sum of the members' own footprints.
     std::vector<Request*> grp{r};
+    std::vector<const Prepared*> gmats{&r->prep};
     int64_t fp = r->footprint;
     pending.pop_front();
     while (!pending.empty() && (int)grp.size() < max_group) {
@@ -135,12 +147,19 @@ void gsofa_server::admit_loop() {
       if (q->cancel_requested) {
         pending.pop_front();
         q->state = State::Cancelled;
+        release_prep(q);
         cv_done.notify_all();
         continue;
       }
-      if (!same_options(q->opt, r->opt) || in_use.load() + fp + q->footprint > budget) break;
+      if (!same_options(q->opt, r->opt)) break;
+      gmats.push_back(&q->prep);
+      int64_t fq = 0;
+      if (DeviceSet::footprint(gmats, r->opt, &fq) != GSOFA_OK || in_use.load() + fq > budget) {
+        gmats.pop_back();
+        break;
+      }
       grp.push_back(q);
-      fp += q->footprint;
+      fp = fq;
       pending.pop_front();
     }
     Slot& s = *free_slot();
@@ -212,6 +231,7 @@ void gsofa_server::finish(Slot& s) {
     }
   }
   s.set.reset();  // stream-ordered frees on the slot
+  for (Request* r : s.group) release_prep(r);  // only n and the results are kept
   const Clock::time_point now = Clock::now();
   std::lock_guard<std::mutex> lk(mu);
   in_use -= s.footprint;
@@ -305,13 +325,23 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   gsofa_options def;
   gsofa_options_default(&def);
   const gsofa_options* o = opt ? opt : &def;
+  // Served requests report totals, nzU and supernode digests; the explicit structure and the
+  // autotuner (which times candidates on the device) are plan and batch features.
+  if (o->structure) return fail(GSOFA_ERR_INVALID_ARG, "the server does not produce the explicit structure");
+  if (o->autotune)
+    return fail(GSOFA_ERR_INVALID_ARG,
+                "the server does not autotune: tune a plan or batch and pass heavy_threads, heavy_wgs_per_cu and "
+                "chain_block");
   r->opt.chunk = o->chunk_size;
   r->opt.n_processes = o->n_processes;
   r->opt.device = srv->device;
   r->opt.heavy_threads = o->heavy_threads;
+  r->opt.heavy_wgs_per_cu = o->heavy_wgs_per_cu;
+  r->opt.light_wgs_per_cu = o->light_wgs_per_cu;
   r->opt.qcap = o->queue_capacity;
   r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 0;
   r->opt.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
+  r->opt.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
   r->opt.structure = 0;
   DeviceInfo info;
   int rc = device_info(srv->device, info);
@@ -319,6 +349,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   r->opt.cu_share = srv->cu_share;  // 0: grids sized for the whole device
   rc = prepare(a, r->prep);
   if (rc != GSOFA_OK) return rc;
+  r->n = r->prep.n;
   rc = (int)DeviceSet::footprint({&r->prep}, r->opt, &r->footprint);
   if (rc != GSOFA_OK) return rc;
   r->t_submit = Clock::now();
@@ -331,6 +362,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
     r->error = GSOFA_ERR_REJECTED;
     r->error_text = "request footprint " + std::to_string(r->footprint) + " B exceeds the HBM budget " +
                     std::to_string(srv->budget) + " B";
+    release_prep(r.get());
     srv->requests[r->id] = std::move(r);
     return GSOFA_OK;
   }
@@ -355,7 +387,7 @@ static int report(Request* r, int n_processes_dummy, gsofa_result* out) {
     out->fill_count = m.fill;
     out->nnz_L = m.nnz_l;
     out->nnz_U = m.nnz_u;
-    out->nnz_F = m.nnz_l + m.nnz_u + (uint64_t)r->prep.n;
+    out->nnz_F = m.nnz_l + m.nnz_u + (uint64_t)r->n;
     out->edge_checks = m.W;
     out->reach = m.R;
     out->supernodes_counted = m.nsup;
@@ -401,9 +433,32 @@ int gsofa_server_cancel(gsofa_server* srv, int64_t id) {
         break;
       }
     r->state = State::Cancelled;
+    release_prep(r);
     srv->cv_done.notify_all();
     srv->cv_admit.notify_all();
   }
+  return GSOFA_OK;
+}
+
+int gsofa_server_release(gsofa_server* srv, int64_t id) {
+  if (!srv) return fail(GSOFA_ERR_INVALID_ARG, "server is null");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->requests.find(id);
+  if (it == srv->requests.end()) return fail(GSOFA_ERR_INVALID_ARG, "unknown request id");
+  const State st = it->second->state;
+  if (st == State::Pending || st == State::Running)
+    return fail(GSOFA_ERR_STATE, "request is still queued or running (cancel it first)");
+  srv->requests.erase(it);
+  return GSOFA_OK;
+}
+
+int gsofa_server_retained(gsofa_server* srv, int64_t* requests, int64_t* host_bytes) {
+  if (!srv) return fail(GSOFA_ERR_INVALID_ARG, "server is null");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  int64_t b = 0;
+  for (const auto& kv : srv->requests) b += (int64_t)sizeof(Request) + prep_bytes(kv.second->prep);
+  if (requests) *requests = (int64_t)srv->requests.size();
+  if (host_bytes) *host_bytes = b;
   return GSOFA_OK;
 }
 
@@ -444,6 +499,7 @@ int gsofa_server_stop(gsofa_server* srv) {
     srv->stopping = true;
     for (Request* r : srv->pending) {
       r->state = State::Cancelled;
+      release_prep(r);
     }
     srv->pending.clear();
     srv->cv_admit.notify_all();

@@ -86,6 +86,18 @@ int gsofa_read_binary(const char* beg_file, const char* csr_file, int32_t index_
 int gsofa_write_binary(const gsofa_csr* a, const char* beg_file, const char* csr_file, int32_t index_bytes,
                        int32_t vert_bytes);
 
+/* The reference's inputs are SuperLU_DIST's reordered matrices (README.md:27-30): SuperLU_DIST
+ * factors Pc * Pr * A * Pc^T. gsofa_permute builds that pattern, B(perm_c[perm_r[i]], perm_c[j]) =
+ * A(i, j); perm_r or perm_c may be null (identity) and must otherwise be permutations of [0, n).
+ * gsofa_read_matrix_market reads a square coordinate Matrix Market file (pattern, real, integer or
+ * complex; general, or symmetric / skew-symmetric / Hermitian expanded to both triangles; values
+ * ignored). gsofa_read_permutation reads n integers (base 0 or 1) and checks the permutation.
+ * The program gsofa_convert turns a .mtx file and permutation files into the ASCII triplet or
+ * binary files. No counterpart in the reference's code (its dump step lives in SuperLU_DIST). */
+int gsofa_read_matrix_market(const char* path, gsofa_host_matrix** out);
+int gsofa_read_permutation(const char* path, int32_t n, int32_t base, int32_t* perm);
+int gsofa_permute(const gsofa_csr* a, const int32_t* perm_r, const int32_t* perm_c, gsofa_host_matrix** out);
+
 /* ---- the drop-in entry ---------------------------------------------------------------------
  * Replaces void symbfact_min_id(int args, char** argv, int myrank, ull_t& fill_count,
  * double& time, int& N_supernodes, ull_t& Process_TEPS)
@@ -174,6 +186,30 @@ int gsofa_structure_size(gsofa_plan* plan, int64_t* nnz_u, int64_t* nnz_l);
 /* Strict U by rows (u_rowptr n+1, u_col nnz_u) and strict L by columns (l_colptr n+1,
  * l_row nnz_l), each row/column sorted ascending. */
 int gsofa_structure_get(gsofa_plan* plan, int64_t* u_rowptr, int32_t* u_col, int64_t* l_colptr, int32_t* l_row);
+/* ---- supernodal export ------------------------------------------------------------------
+ * The supernode partition the reference computes in struct Super {start, end}
+ * (gSoFa_SOAComparison/symbfact_warp_centric_new_scheduling.cuh:341-345, written at :1970-1971
+ * and :2039, never returned): supernode k is the row range [xsup[k], xsup[k+1]), starting at a row
+ * where the supernode count increments (flag 1) and ending before the next one; xsup[nsuper] = n.
+ * The count equals gsofa_result.supernodes_counted. xsup (nsuper + 1 entries) and supno (n
+ * entries: the supernode of each row) may be null; call once with both null to get nsuper. */
+int gsofa_supernode_partition(gsofa_plan* plan, int64_t* nsuper, int32_t* xsup, int32_t* supno);
+/* Per-supernode panel structure for a supernodal numeric factorisation (requires the structure
+ * option and a completed run). For supernode k = [a, b): lsub[xlsub[k] .. xlsub[k+1]) = sorted
+ * rows i >= b with (i, j) in F for some column j in [a, b) (the L panel below the diagonal
+ * block); usub[xusub[k] .. xusub[k+1]) = sorted columns m >= b with (i, m) in F for some row i
+ * in [a, b) (the U panel). The diagonal block is dense in that format and not listed. Sizes first
+ * (nsuper, lsub and usub lengths), then the arrays (xsup nsuper + 1, xlsub and xusub nsuper + 1).
+ * Built on the host from the structure. No counterpart in the reference beyond Super. */
+int gsofa_supernodal_structure_size(gsofa_plan* plan, int64_t* nsuper, int64_t* nnz_lsub, int64_t* nnz_usub);
+int gsofa_supernodal_structure_get(gsofa_plan* plan, int32_t* xsup, int64_t* xlsub, int32_t* lsub, int64_t* xusub,
+                                   int32_t* usub);
+/* The same from caller arrays (no device): supernode flags (n bytes) and the strict structure in
+ * gsofa_structure_get's form. With every output array null only the sizes are written. */
+int gsofa_supernodal_from_structure(int32_t n, const uint8_t* flags, const int64_t* u_rowptr, const int32_t* u_col,
+                                    const int64_t* l_colptr, const int32_t* l_row, int64_t* nsuper, int64_t* nnz_lsub,
+                                    int64_t* nnz_usub, int32_t* xsup, int64_t* xlsub, int32_t* lsub, int64_t* xusub,
+                                    int32_t* usub);
 /* Device time of the last run's stages, from events on the run's stream: ms[0] preparation
  * (ELL packing + transpose), ms[1] the wave (hash) path, ms[2] the workgroup (bitmap) path,
  * ms[3] supernode detection. No counterpart in the reference (its only timers are wtime()). */
@@ -201,6 +237,8 @@ int gsofa_batch_sync(gsofa_batch* batch);
 int gsofa_batch_result_get(gsofa_batch* batch, int32_t index, gsofa_result* out);
 int gsofa_batch_nzu_get(gsofa_batch* batch, int32_t index, int32_t* nzu);
 int gsofa_batch_supernodes_get(gsofa_batch* batch, int32_t index, uint8_t* flags);
+/* gsofa_supernode_partition for batch member `index`. */
+int gsofa_batch_supernode_partition(gsofa_batch* batch, int32_t index, int64_t* nsuper, int32_t* xsup, int32_t* supno);
 void gsofa_batch_destroy(gsofa_batch* batch);
 
 /* ---- serving: continuous batching of requests ------------------------------------------
@@ -221,7 +259,10 @@ typedef struct gsofa_server_options {
 void gsofa_server_options_default(gsofa_server_options* opt);
 int gsofa_server_start(const gsofa_server_options* opt, gsofa_server** out);
 /* Copies and prepares the pattern on the calling thread, then queues it. opt: chunk_size,
- * n_processes and tuning fields are used; structure is not produced by the server. */
+ * n_processes and the tuning fields (heavy_threads, heavy_wgs_per_cu, light_wgs_per_cu,
+ * queue_capacity, chain_block, pair_directions, bitmap_in_hbm) are used; requests with different
+ * options are never grouped. structure = 1 and autotune = 1 are plan/batch features and return
+ * GSOFA_ERR_INVALID_ARG. */
 int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_options* opt, int64_t* request_id);
 /* GSOFA_OK with the result, or GSOFA_ERR_NOT_READY / _CANCELLED / _REJECTED / another error. */
 int gsofa_server_poll(gsofa_server* srv, int64_t request_id, gsofa_result* out);
@@ -229,6 +270,13 @@ int gsofa_server_poll(gsofa_server* srv, int64_t request_id, gsofa_result* out);
 int gsofa_server_wait(gsofa_server* srv, int64_t request_id, double timeout_s, gsofa_result* out);
 /* Pending requests are dropped; running ones finish on the device and report GSOFA_ERR_CANCELLED. */
 int gsofa_server_cancel(gsofa_server* srv, int64_t request_id);
+/* Forgets a finished, failed, cancelled or rejected request (its id becomes unknown); a queued or
+ * running one returns GSOFA_ERR_STATE. A finished request keeps only its totals and digests (its
+ * pattern is released when it leaves the device); a long-running server releases ids it has
+ * read to keep its host memory bounded. */
+int gsofa_server_release(gsofa_server* srv, int64_t request_id);
+/* Requests the server still holds (any state) and the host bytes they keep alive. */
+int gsofa_server_retained(gsofa_server* srv, int64_t* requests, int64_t* host_bytes);
 /* Submit-to-result latency, time queued before admission, and FNV-1a digests of nzU and the
  * supernode flags of a finished request. */
 int gsofa_server_request_info(gsofa_server* srv, int64_t request_id, double* latency_ms, double* queue_ms,

@@ -303,6 +303,62 @@ def test_entry_and_cli_on_reference_files(tmp_path):
     assert csv[1].startswith(files[1] + ";1;0;0;1;")  # argv[2];argv[4];argv[6];argv[7];argv[8]
 
 
+@pytest.mark.parametrize("case", ["ds_11948", "grid2d_4096"])
+def test_supernodal_export(case):
+    """The supernode partition and panels of a plan against the host export of the oracle's F and
+    flags (the host export itself is pinned against the dense definition on the CPU)."""
+    p = load_ds() if case == "ds_11948" else g.generate_workload(g.WL_GRID2D_4096)
+    say(f"[supernodal] {case} n={p.n}")
+    o = oracle.symbfact(p, chunk=128, structure=True)
+    ref = g.supernodal_from_structure(o["sup_flag"], *oracle.split_structure_fast(p.n, o["F_rowptr"], o["F_cols"]))
+    with g.Plan(p, chunk_size=128, structure=1) as plan:
+        plan.run()
+        xsup, supno = plan.supernode_partition()
+        got = plan.supernodal_structure()
+        assert len(xsup) - 1 == plan.result().supernodes_counted == o["nsup"]
+    assert np.array_equal(xsup, ref[0])
+    assert np.array_equal(supno, np.repeat(np.arange(len(xsup) - 1), np.diff(xsup)))
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("ib,vb", [(4, 4), (8, 8), (8, 4)])
+def test_binary_files_through_the_engine(tmp_path, ib, vb):
+    """The reference's dataset written as binary CSR (gSoFa_SOAComparison/graph.hpp layout) and read
+    back runs through the engine: 980,136 fill-ins and the oracle's nzU and supernode flags."""
+    ds = load_ds()
+    beg, csr = str(tmp_path / "beg.bin"), str(tmp_path / "csr.bin")
+    g.write_binary(ds, beg, csr, ib, vb)
+    p = g.read_binary(beg, csr, ib, vb)
+    say(f"[binary] widths {ib}/{vb} n={p.n} nnz={p.nnz}")
+    o = oracle.symbfact(ds, chunk=128)
+    with g.Plan(p, chunk_size=128) as plan:
+        plan.run()
+        r = plan.result()
+        assert r.fill_count == 980136 == o["fill"]
+        assert r.supernodes_counted == o["nsup"]
+        assert np.array_equal(plan.nzu(), o["nzu"])
+        assert np.array_equal(plan.supernode_flags(), o["sup_flag"])
+
+
+def test_results_before_any_run_are_a_state_error():
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    with g.Plan(p) as plan:
+        for call in (plan.result, plan.nzu, plan.supernode_flags, plan.stage_times, plan.sync):
+            with pytest.raises(g.GsofaError) as e:
+                call()
+            assert e.value.code == g.ERR_STATE
+    b = g.Batch([p, p])
+    try:
+        with pytest.raises(g.GsofaError) as e:
+            b.result(0)
+        assert e.value.code == g.ERR_STATE
+        b.run().sync()
+        assert b.result(1).fill_count == b.result(0).fill_count
+    finally:
+        b.close()
+
+
 def _large(name):
     import hashlib
     import json
@@ -337,10 +393,27 @@ def test_full_size_against_committed_oracle_values(name, wl):
             say(f"[full size] {name} hbm bitmaps: {r.time_ms:.2f} ms, stages {plan.stage_times()}")
 
 
+@pytest.mark.parametrize("name,wl", [("grid3d_110592", g.WL_GRID3D_110592), ("grid2d_659033", g.WL_GRID2D_659033)])
+def test_full_size_structure_equals_oracle(name, wl):
+    """The explicit structure at full size, against digests of the oracle's F (strict U by rows,
+    strict L by columns) committed by tests/golden/make_golden_large.py."""
+    gd, _ = _large(name)
+    if "U_rows_sha256" not in gd:
+        pytest.skip(f"no committed structure digests for {name}")
+    p = g.generate_workload(wl)
+    say(f"[full size structure] {name} n={p.n}")
+    with g.Plan(p, chunk_size=128, structure=1) as plan:
+        plan.run()
+        r = plan.result()
+        up, ui, lp, li = plan.structure()
+    assert (int(up[-1]), int(lp[-1])) == (gd["nnz_U"], gd["nnz_L"]) == (r.nnz_U, r.nnz_L)
+    assert oracle.structure_digests(up, ui, lp, li) == (gd["U_rows_sha256"], gd["L_cols_sha256"])
+
+
 def test_full_size_structure_properties():
-    """At 659,033 rows the oracle cannot produce F in test time; check size-independent
-    properties of the explicit structure instead: counts agree with the per-row counts, rows are
-    sorted and duplicate-free, every entry of A is in F, and runs with different tuning agree."""
+    """Size-independent properties of the explicit structure at 659,033 rows: counts agree with
+    the per-row counts, rows are sorted and duplicate-free, every entry of A is in F, and runs
+    with different tuning agree."""
     p = g.generate_workload(g.WL_GRID2D_659033)
     with g.Plan(p, structure=1) as plan:
         plan.run()

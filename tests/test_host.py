@@ -235,3 +235,114 @@ def test_no_device_is_an_error_code_not_a_fallback():
 def test_cli_usage():
     r = subprocess.run([g.CLI_PATH], capture_output=True, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+# ---- conversion: Matrix Market and the SuperLU_DIST-style reordering -------------------------
+
+def dense_of(p):
+    D = np.zeros((p.n, p.n), np.int32)
+    for v in range(p.n):
+        np.add.at(D[v], p.adj[p.beg[v]:p.end[v]], 1)
+    return D
+
+
+def write_mtx(path, D, field="pattern", symmetry="general", lower_only=False):
+    rows, cols = np.nonzero(np.tril(D) if lower_only else D)
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate {field} {symmetry}\n% a comment\n")
+        f.write(f"{D.shape[0]} {D.shape[1]} {len(rows)}\n")
+        for i, j in zip(rows, cols):
+            val = {"pattern": "", "real": " 1.5e-3", "integer": " 7", "complex": " 1.0 -2.0"}[field]
+            f.write(f"{i + 1} {j + 1}{val}\n")
+
+
+@pytest.mark.parametrize("field", ["pattern", "real", "integer", "complex"])
+def test_matrix_market_general(tmp_path, field):
+    rng = np.random.default_rng(1)
+    D = (rng.random((50, 50)) < 0.08).astype(np.int32)
+    write_mtx(tmp_path / "a.mtx", D, field)
+    p = g.read_matrix_market(str(tmp_path / "a.mtx"))
+    assert p.n == 50 and np.array_equal(dense_of(p), D)
+
+
+def test_matrix_market_symmetric_expands(tmp_path):
+    rng = np.random.default_rng(2)
+    D = (rng.random((40, 40)) < 0.1).astype(np.int32)
+    D = ((D + D.T) > 0).astype(np.int32)
+    write_mtx(tmp_path / "s.mtx", D, "real", "symmetric", lower_only=True)
+    p = g.read_matrix_market(str(tmp_path / "s.mtx"))
+    assert np.array_equal(dense_of(p), D)
+
+
+@pytest.mark.parametrize("text,code", [
+    ("%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n", g.ERR_IO),
+    ("2 2 1\n1 1\n", g.ERR_IO),
+    ("%%MatrixMarket matrix coordinate pattern general\n2 3 1\n1 1\n", g.ERR_IO),
+    ("%%MatrixMarket matrix coordinate pattern general\n2 2 1\n3 1\n", g.ERR_BAD_CSR),
+    ("%%MatrixMarket matrix coordinate pattern general\n2 2 2\n1 1\n", g.ERR_IO),
+])
+def test_matrix_market_errors(tmp_path, text, code):
+    (tmp_path / "e.mtx").write_text(text)
+    with pytest.raises(g.GsofaError) as e:
+        g.read_matrix_market(str(tmp_path / "e.mtx"))
+    assert e.value.code == code
+
+
+def test_permute_is_superlu_dist_reordering():
+    """B = Pc Pr A Pc^T with B(perm_c[perm_r[i]], perm_c[j]) = A(i, j): checked as dense matrix
+    products with permutation matrices (P[perm[i], i] = 1)."""
+    rng = np.random.default_rng(3)
+    n = 60
+    D = (rng.random((n, n)) < 0.07).astype(np.int64)
+    p = g.Pattern.from_dense(D.astype(bool))
+    pr, pc = rng.permutation(n).astype(np.int32), rng.permutation(n).astype(np.int32)
+    Pr = np.zeros((n, n), np.int64)
+    Pr[pr, np.arange(n)] = 1
+    Pc = np.zeros((n, n), np.int64)
+    Pc[pc, np.arange(n)] = 1
+    B = dense_of(g.permute(p, pr, pc))
+    assert np.array_equal(B, Pc @ Pr @ D @ Pc.T)
+    assert np.array_equal(dense_of(g.permute(p, pr, None)), Pr @ D)
+    assert np.array_equal(dense_of(g.permute(p, None, None)), D)
+    inv = np.argsort(pc).astype(np.int32)  # Pc^T Pc = I: permuting back restores A
+    assert np.array_equal(dense_of(g.permute(g.permute(p, None, pc), None, inv)), D)
+    with pytest.raises(g.GsofaError) as e:
+        g.permute(p, np.zeros(n, np.int32), None)
+    assert e.value.code == g.ERR_INVALID_ARG
+
+
+def test_convert_tool_reference_dataset(tmp_path):
+    """The reference's input as a Matrix Market file through gsofa_convert with identity
+    permutations gives the reference's ASCII triplet back; with a symmetric reordering the
+    converted matrix is P A P^T."""
+    import oracle
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ds_11948.npz"))
+    ds = g.Pattern(int(len(z["beg"])), z["beg"], z["end"], z["adj"])
+    rows = np.repeat(np.arange(ds.n), ds.end - ds.beg)
+    cols = np.concatenate([ds.adj[b:e] for b, e in zip(ds.beg, ds.end)])
+    with open(tmp_path / "ds.mtx", "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate pattern general\n{ds.n} {ds.n} {len(rows)}\n")
+        f.write("".join(f"{i + 1} {j + 1}\n" for i, j in zip(rows, cols)))
+    ident = tmp_path / "ident.txt"
+    ident.write_text(" ".join(str(i + 1) for i in range(ds.n)))
+    r = subprocess.run([g.CONVERT_PATH, str(tmp_path / "ds.mtx"), str(tmp_path / "out"), "--perm-r", str(ident),
+                        "--perm-c", str(ident), "--one-based"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    q = g.read_ascii(*(str(tmp_path / f"out.{x}") for x in ("beg", "end", "csr")))
+    assert q.n == ds.n and q.nnz == ds.nnz
+    o = oracle.symbfact(q, chunk=128)
+    assert o["fill"] == 980136
+    perm = np.random.default_rng(4).permutation(ds.n)
+    (tmp_path / "perm.txt").write_text(" ".join(map(str, perm)))
+    r = subprocess.run([g.CONVERT_PATH, str(tmp_path / "ds.mtx"), str(tmp_path / "b"), "--perm-c",
+                        str(tmp_path / "perm.txt"), "--binary", "8", "4"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    b = g.read_binary(str(tmp_path / "b.beg.bin"), str(tmp_path / "b.csr.bin"), 8, 4)
+    ref = g.permute(ds, None, perm.astype(np.int32))
+    for v in range(0, ds.n, 97):
+        assert np.array_equal(np.sort(b.adj[b.beg[v]:b.end[v]]), np.sort(ref.adj[ref.beg[v]:ref.end[v]]))
+
+
+def test_convert_tool_usage():
+    r = subprocess.run([g.CONVERT_PATH], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
