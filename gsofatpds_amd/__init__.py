@@ -152,6 +152,7 @@ _SIGNATURES = {
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64)]),
     "gsofa_server_stop": (C.c_int, [C.c_void_p]),
+    "gsofa_digest_fnv1a": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_uint64)]),
 }
 
 
@@ -629,8 +630,15 @@ class Server:
 
 
 def fnv1a(data: bytes) -> int:
-    """The digest the server reports for nzU and the supernode flags."""
-    h = 1469598103934665603
-    for b in data:
-        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
-    return h
+    """The digest the server reports for nzU and the supernode flags (gsofa_digest_fnv1a)."""
+    out = C.c_uint64()
+    _check(lib().gsofa_digest_fnv1a(data, len(data), C.byref(out)))
+    return out.value
+
+
+def fnv1a_array(a: np.ndarray) -> int:
+    """fnv1a of an array's bytes (int32 nzU, uint8 flags), without a copy."""
+    a = np.ascontiguousarray(a)
+    out = C.c_uint64()
+    _check(lib().gsofa_digest_fnv1a(a.ctypes.data_as(C.c_void_p), a.nbytes, C.byref(out)))
+    return out.value

@@ -103,4 +103,16 @@ int gsofa_host_matrix_csr(const gsofa_host_matrix* m, gsofa_csr* view) {
 
 void gsofa_host_matrix_free(gsofa_host_matrix* m) { delete m; }
 
+int gsofa_digest_fnv1a(const void* data, int64_t bytes, uint64_t* out) {
+  if (!out || bytes < 0 || (bytes > 0 && !data)) return fail(GSOFA_ERR_INVALID_ARG, "bad digest arguments");
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  uint64_t h = 1469598103934665603ull;
+  for (int64_t i = 0; i < bytes; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  *out = h;
+  return GSOFA_OK;
+}
+
 }  // extern "C"

@@ -281,6 +281,8 @@ int gsofa_server_retained(gsofa_server* srv, int64_t* requests, int64_t* host_by
  * supernode flags of a finished request. */
 int gsofa_server_request_info(gsofa_server* srv, int64_t request_id, double* latency_ms, double* queue_ms,
                               uint64_t* nzu_digest, uint64_t* supernode_digest);
+/* The FNV-1a digest (64-bit) gsofa_server_request_info reports, of any byte range. */
+int gsofa_digest_fnv1a(const void* data, int64_t bytes, uint64_t* out);
 int gsofa_server_stats(gsofa_server* srv, int64_t* completed, int32_t* running, int32_t* peak_running,
                        int64_t* pending, int64_t* hbm_in_use, int64_t* hbm_budget);
 /* Drops pending requests, waits for running ones, releases everything. */

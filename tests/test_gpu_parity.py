@@ -341,6 +341,45 @@ def test_binary_files_through_the_engine(tmp_path, ib, vb):
         assert np.array_equal(plan.supernode_flags(), o["sup_flag"])
 
 
+def test_batch256_equals_plain_and_oracle():
+    """The full 256-member batch (n log-uniform in [2,000, 200,000], half ND stencils, half
+    Erdos-Renyi in RCM order) in one captured graph at the bench's tuned shape, replayed twice:
+    every member's totals, nzU and supernode flags equal a plain plan's, and members below 12,000
+    rows equal the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(8) as ex:
+        pats = list(ex.map(lambda k: g.generate_workload(g.WL_BATCH256, k), range(256)))
+    say(f"[batch256] n total {sum(p.n for p in pats)}")
+    keys = ("fill_count", "nnz_F", "edge_checks", "reach", "supernodes_counted")
+    b = g.Batch(pats, heavy_threads=256, heavy_wgs_per_cu=4, chain_block=768)
+    try:
+        b.run().sync()
+        first = [tuple(getattr(b.result(i), k) for k in keys) for i in range(256)]
+        b.run().sync()
+        res = [tuple(getattr(b.result(i), k) for k in keys) for i in range(256)]
+        assert res == first  # replays agree
+        nzu = [b.nzu(i) for i in range(256)]
+        flags = [b.supernode_flags(i) for i in range(256)]
+    finally:
+        b.close()
+    say("[batch256] batch done; plain plans and oracle")
+    checked = 0
+    for i, p in enumerate(pats):
+        with g.Plan(p) as plan:
+            plan.run()
+            r = plan.result()
+            assert tuple(getattr(r, k) for k in keys) == res[i], i
+            assert np.array_equal(plan.nzu(), nzu[i]), i
+            assert np.array_equal(plan.supernode_flags(), flags[i]), i
+        if p.n < 12000:
+            o = oracle.symbfact(p, chunk=128)
+            assert res[i] == (o["fill"], o["nnz_F"], o["W"], o["R"], o["nsup"]), i
+            assert np.array_equal(nzu[i], o["nzu"]) and np.array_equal(flags[i], o["sup_flag"]), i
+            checked += 1
+    say(f"[batch256] all members equal plain; {checked} checked against the oracle")
+    assert checked >= 90
+
+
 def test_results_before_any_run_are_a_state_error():
     p = g.generate_workload(g.WL_GRID2D_4096)
     with g.Plan(p) as plan:
