@@ -126,7 +126,8 @@ int DeviceSet::alloc(T*& p, size_t count, const char* what) {
 // fit in a workgroup's LDS next to the static part and the smallest queues.
 static bool place_bitmaps_in_hbm(int32_t n_max, const Options& opt, const DeviceInfo& info) {
   const size_t bitmap_bytes = (((size_t)n_max + 31) / 32) * 4;
-  return opt.bitmap_in_hbm == 1 || bitmap_bytes + kHeavyLdsReserve + 256 * 8 > info.lds_per_block;
+  return opt.bitmap_in_hbm == 1 ||
+         bitmap_bytes + kHeavyLdsReserve + heavy_stash_bytes(true) + 256 * 8 > info.lds_per_block;
 }
 
 int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream) {
@@ -165,7 +166,8 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
 
   // ---- launch configuration ----
   bw_max_ = (n_max_ + 31) / 32;
-  const size_t reserve = kHeavyLdsReserve;
+  // static LDS and the window's rows (per direction) besides the bitmap, queues and U lists
+  const size_t reserve = kHeavyLdsReserve + heavy_stash_bytes(paired_any_ != 0);
   // Bitmaps in HBM when asked for, or when one does not fit in a workgroup's LDS next to the
   // smallest queues (n above about 1.2 M rows): slower claims (L2 atomics), no size limit.
   hbm_bitmap_ = place_bitmaps_in_hbm(n_max_, opt, info);
@@ -205,7 +207,9 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     qcap_ = (int32_t)(q & ~int64_t(63));
   }
   const size_t lds = heavy_lds_bytes(bw_max_, qcap_, hbm_bitmap_, ucap_, paired_any_ != 0);
-  if (lds + reserve > info.lds_per_block) return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
+  // lds holds the window's rows already; the static part is the rest
+  if (lds + kHeavyLdsReserve > info.lds_per_block)
+    return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
   GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds, hbm_bitmap_), "hipFuncSetAttribute(k_heavy)");
   const int occ = heavy_occupancy(heavy_threads_, lds, hbm_bitmap_);
   if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");

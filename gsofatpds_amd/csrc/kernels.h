@@ -98,8 +98,15 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
                               int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s);
 // LDS of k_heavy outside its dynamic part (static shared state, 2.6 KB) plus margin.
 constexpr size_t kHeavyLdsReserve = 3072;
-// Dynamic LDS of k_heavy: the bitmap (unless it lives in HBM), two frontier queue levels, and
-// the first ucap entries of the staged U list of each direction (two with paired items).
+// Rows of a block whose facts (and ELL rows) k_heavy reads into LDS in one round.
+constexpr int kHeavyWin = 64;
+// The window's ELL rows (plus one look-ahead row) kept in LDS per direction.
+constexpr size_t heavy_stash_bytes(bool paired_any) {
+  return (size_t)(kHeavyWin + 1) * kEll * 4 * (paired_any ? 2 : 1);
+}
+// Dynamic LDS of k_heavy: the bitmap (unless it lives in HBM), two frontier queue levels, the
+// first ucap entries of the staged U list of each direction (two with paired items), and the
+// window's rows.
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap, int32_t ucap, bool paired_any);
 hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap);
 // Saved bitmaps per direction in a workgroup region when some matrix runs paired items: a ring

@@ -107,6 +107,20 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
+// Lanes of one wave hand data to each other through LDS (one lane writes a count, others read
+// it; lanes read queue entries other lanes wrote) between steps that no workgroup barrier
+// separates. Release and acquire at wavefront scope around a wave barrier: every access of the
+// step before is ordered before every access after, and no shared value is carried over in a
+// register.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A wave-uniform value into a scalar register.
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
 __device__ __forceinline__ int find_segment(const int64_t* base, int nseg, int64_t k) {
   // base[0] = 0 <= ... <= base[nseg]; returns b with base[b] <= k < base[b + 1]
   int lo = 0, hi = nseg;
@@ -808,12 +822,43 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
 // n-bit bitmap does not fit in LDS). Its claims are then L2 atomics, its reads bypass L1, and a
 // wave's plain stores to it (clears, copies) are acknowledged by L2 before each barrier.
 //
-// The kernel is bound by dependent global-memory round trips, one workgroup per CU: the
-// per-row facts (escalated? holds its predecessor? structure offsets) are read for a window
-// of up to kHeavyWin rows in one round into LDS, loads of one expansion step are issued
-// together, and bitmap copies move kImportBatch words per thread per round.
-constexpr int kHeavyWin = 64;
+// Work split inside the workgroup. Wave 0, the leader, runs the rows: it plans each row (chain
+// base, early tests, import), runs the seed pass and every BFS level of at most kSoloMax
+// vertices by itself, and finishes the row (counts, window bits, the U list into the
+// structure). The other waves wait at one barrier and join only when the leader asks: to clear,
+// copy in, OR in or save a bitmap, and for levels wider than kSoloMax vertices. Most escalated
+// rows of structured patterns chain from their predecessor and reach nothing new below
+// themselves (their traversal is the seed row alone): they cost the leader a few hundred
+// cycles and the workgroup no barrier. A request is the command word in LDS, written by the
+// leader before a barrier and read by every wave after it, so every wave takes the same
+// barriers. The per-row facts of a window of up to kHeavyWin rows (escalated? holds its
+// predecessor? structure offsets, and the rows themselves) are read in one round into LDS.
 constexpr int kImportBatch = 8;
+constexpr int kSoloNarrow = 4;  // levels of at most this many vertices: 16 lanes per vertex
+constexpr int kSoloMax = 64;    // levels of at most this many vertices: the leader alone
+
+enum : int32_t { kCmdDone = 0, kCmdClear = 1, kCmdCopyIn = 2, kCmdImport = 3, kCmdSave = 4, kCmdWide = 5 };
+
+// The leader's request: written by its lane 0 before a barrier, read by every thread after it.
+struct HeavyCtl {
+  int32_t cmd, d, src, level;
+  int32_t imported, slot;  // slot: the saved bitmap to copy in, OR in or write
+  uint32_t ubase;          // carried U-list entries (the row's new entries follow them)
+  int32_t has_prev;
+};
+
+// The leader's state between its turns, in LDS: a turn starts from it and writes it back before
+// the barrier that ends the turn, so no register of the leader is live while the workgroup works.
+struct LeaderState {
+  int32_t prev[2];      // per direction: the last row traversed in this block
+  int32_t sv[2][kSavedRing];  // per direction: the row each saved bitmap holds
+  int32_t wp[2];        // per direction: the ring slot of the next save
+  int32_t lds_row, lds_dir;   // what the LDS bitmap holds
+  int32_t li_next, di_next, phase;  // cursor in the window; 0 start a row, 1 bitmap ready, 2 levels, 3 finish
+  int32_t d, li, fi, slot1, tslot, level, src, t, chunk_end;  // the row in progress
+  int32_t chained, has_prev, imported;
+  uint32_t ubase, gap;
+};
 
 // kPairedAny: some matrix of the set runs paired items (otherwise that code is compiled out).
 template <int kThreads, bool kPairedAny, bool kHbmBitmap>
@@ -825,30 +870,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                     uint32_t* __restrict__ hbm_bitmaps, int32_t ucap) {
   static_assert(kEll == 16, "the window scan maps one row to 16 lanes");
   extern __shared__ uint32_t dyn[];
-  constexpr int kWaves = kThreads / 64;
   constexpr int kPfRounds = ((kHeavyWin + 1) * kEll + kThreads - 1) / kThreads;
+  constexpr int kStash = (kHeavyWin + 1) * kEll;  // ELL slots of a window's rows, per direction
   __shared__ unsigned long long s_idx;
-  __shared__ int32_t s_cnt[3];
-  __shared__ uint32_t s_ust, s_unew, s_useed;
-  __shared__ unsigned long long s_wall, s_wovl, s_xe;
-  __shared__ uint32_t s_rall, s_rovl;
-  __shared__ int32_t s_deg0, s_import, s_src_was_set, s_early;
-  __shared__ uint32_t s_gap;  // X_t's vertices between the chain base t and src
-  __shared__ int32_t s_drop;  // src was found (and removed) in the carried U list
-  __shared__ uint32_t s_scan[kWaves];
+  __shared__ int32_t s_cnt[3];     // queue lengths: level L at index L % 3
+  __shared__ uint32_t s_ust;       // U-list entries appended after the carried ones
+  __shared__ uint32_t s_unew;      // new U entries found by cooperative phases
+  __shared__ unsigned long long s_wall, s_wovl, s_xe;  // row sums of earlier turns and cooperative phases
+  __shared__ uint32_t s_rall, s_rovl, s_useed, s_deg0;
+  __shared__ int32_t s_import, s_src_was_set;
   __shared__ int32_t s_esc[2][kHeavyWin + 1];   // row w0 + i was escalated (per direction)
   __shared__ int32_t s_hold[2][kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
   __shared__ int64_t s_lp[2][kHeavyWin + 2];    // structure offsets of rows w0 .. w0 + np
+  __shared__ HeavyCtl s_ctl;
+  __shared__ LeaderState s_ls;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid >> 6;
+  const bool leader = wave == 0;
   // the n-bit bitmap: LDS, or this workgroup's bw_max words of hbm_bitmaps
   uint32_t* const bm = kHbmBitmap ? hbm_bitmaps + (int64_t)blockIdx.x * bw_max : dyn;
   int32_t* const ql0 = reinterpret_cast<int32_t*>(kHbmBitmap ? dyn : dyn + bw_max);
   // the staged U list of each direction: its first ucap entries in LDS, the rest in ustage
   int32_t* const ul0 = ql0 + 2 * qcap;
   int32_t* const ul1 = kPairedAny ? ul0 + ucap : ul0;
+  // the window's rows in ELL form, per direction (the seed rows of the leader)
+  int32_t* const stash0 = ul0 + (kPairedAny ? 2 : 1) * ucap;
+  int32_t* const stash1 = kPairedAny ? stash0 + kStash : stash0;
   // HBM bitmap: its claims are L2 atomics, so its reads bypass L1 (agent-scope loads, never a
   // stale line), and each wave's plain stores (clears, copies) are acknowledged by L2 before a
   // barrier lets another wave's atomics at the same words.
@@ -878,15 +927,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     return kPairedAny ? gsave + (int64_t)(kSavedRing * dd + k) * nbw : gsave;
   };
   // Per direction: the rows of the saved bitmaps (index k < kSavedRing) and the last row
-  // traversed (index kSavedRing). Written by thread 0 after a row's last barrier.
+  // traversed (index kSavedRing). Written by the leader when it finishes a row.
   __shared__ RowFacts s_facts[2][kSavedRing + 1];
 
   int last_mat = -1;
   __shared__ Totals tot;  // thread 0 only (in LDS: keeps the registers for the traversal)
-  if (tid == 0) {
-    tot = Totals{};
-    s_drop = 0;
-  }
+  if (tid == 0) tot = Totals{};
 
   const unsigned long long total = ws->heavy_count;
   while (true) {
@@ -907,24 +953,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int32_t r1 = min(n, r0 + M.chain_block);
     const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
-    // per direction (index = direction; a plain item uses its own direction's slot): the last
-    // row traversed in this block, its W_R, input degree, |R|, |U| and staged U-list length, and
-    // which rows the two saved bitmaps of the direction hold (slot = row parity)
-    int32_t prev0 = -1, prev1 = -1;
-    int32_t sv00 = -1, sv01 = -1, sv02 = -1, sv10 = -1, sv11 = -1, sv12 = -1;  // sv<dir><k>: row in slot k
-    int wp0 = 0, wp1 = 0;                // the slot the next save of the direction takes
-    int32_t lds_row = -1, lds_dir = -1;  // what the LDS bitmap holds
-#define GSOFA_SV(dd, k) ((dd) ? ((k) == 0 ? sv10 : (k) == 1 ? sv11 : sv12) : ((k) == 0 ? sv00 : (k) == 1 ? sv01 : sv02))
+    // The leader's state of this item, per direction (index = direction; a plain item uses its
+    // own direction's slot): the last row traversed in this block, which rows the saved bitmaps
+    // hold, the ring slot of the next save, and what the LDS bitmap holds. Set before the first
+    // window's barrier.
+    if (tid == 0) {
+      for (int dd = 0; dd < 2; ++dd) {
+        s_ls.prev[dd] = -1;
+        s_ls.wp[dd] = 0;
+        for (int k = 0; k < kSavedRing; ++k) s_ls.sv[dd][k] = -1;
+      }
+      s_ls.lds_row = s_ls.lds_dir = -1;
+    }
+#define GSOFA_SV(dd, k) (s_ls.sv[dd][k])
 #define GSOFA_SLOT(dd, r)                                                             \
   ((r) < 0 ? -1                                                                       \
-   : (dd)  ? (sv10 == (r) ? 0 : (kRing > 1 && sv11 == (r)) ? 1 : (kRing > 2 && sv12 == (r)) ? 2 : -1) \
-           : (sv00 == (r) ? 0 : (kRing > 1 && sv01 == (r)) ? 1 : (kRing > 2 && sv02 == (r)) ? 2 : -1))
+   : s_ls.sv[dd][0] == (r) ? 0 : (kRing > 1 && s_ls.sv[dd][1] == (r)) ? 1 : (kRing > 2 && s_ls.sv[dd][2] == (r)) ? 2 : -1)
     for (int32_t w0 = r0; w0 < r1; w0 += kHeavyWin) {
       const int32_t we = min(r1, w0 + kHeavyWin);
       const int32_t np = min(r1, we + 1) - w0;  // rows read: the window plus one look-ahead row
+      if (tid == 0) {
+        s_ls.li_next = 0;
+        s_ls.di_next = 0;
+        s_ls.phase = 0;
+      }
       // ---- window facts, one round of loads per direction ----
       for (int di = 0; di < ndir; ++di) {
         const int dd = paired ? di : item_dir;
+        int32_t* const stash = (kPairedAny && dd) ? stash1 : stash0;
         int32_t esc_v = 0;
         int64_t lp_v = 0;
         if (tid < np) esc_v = gload(M.esc[dd] + w0 + tid);
@@ -946,529 +1002,827 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           }
           const uint64_t b = __ballot(hit);
           if ((i & (kEll - 1)) == 0 && i < np * kEll) s_hold[dd][i >> 4] = ((b >> (lane & 48)) & 0xFFFFull) != 0;
+          if (i < np * kEll) stash[i] = ev[q];
         }
         if (tid < np) s_esc[dd][tid] = esc_v;
         if (want_list && tid <= np) s_lp[dd][tid] = lp_v;
       }
       bsync();
 
-      for (int32_t src = w0; src < we; ++src) {
-        const int32_t li = src - w0;
-        for (int di = 0; di < ndir; ++di) {
-          const int d = paired ? di : item_dir;
-          if (s_esc[d][li] == 0) continue;  // the wave path finished this row
-          const int32_t chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
-          int32_t* const ustage = d ? ustage1 : ustage0;
-          int32_t* const ul = d ? ul1 : ul0;
-          auto uget = [&](uint32_t i) -> int32_t {
-            return (int32_t)i < ucap ? ul[i] : gload(ustage + ((int32_t)i - ucap));
-          };
+      // ---- the window's rows: the leader runs them, the workgroup helps on request ----
+      while (true) {
+        if (leader) {
+          // a turn of the leader: its state from LDS (uniform within the wave)
+          int li_next = s_ls.li_next, di_next = s_ls.di_next;
+          int phase = s_ls.phase;
+          int d = s_ls.d, li = s_ls.li, fi = s_ls.fi, slot1 = s_ls.slot1, tslot = s_ls.tslot, level = s_ls.level;
+          int32_t src = s_ls.src, t = s_ls.t, chunk_end = s_ls.chunk_end;
+          bool chained = s_ls.chained != 0, has_prev = s_ls.has_prev != 0, imported = s_ls.imported != 0;
+          uint32_t ubase = s_ls.ubase, gap = s_ls.gap;
+          // per-lane row sums of this turn's passes (earlier turns' are in the LDS sums)
+          uint32_t w_all = 0, w_ovl = 0, x_e = 0, r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0, deg0 = 0;
+          int32_t cmd = -1;
+          int32_t cslot = 0;
+          while (cmd < 0) {
+            // the staged U list of the row's direction (d changes when a row starts)
+            auto uget = [&](uint32_t i) -> int32_t {
+              return (int32_t)i < ucap ? (d ? ul1 : ul0)[i] : gload((d ? ustage1 : ustage0) + ((int32_t)i - ucap));
+            };
+            auto uput = [&](uint32_t i, int32_t v) {
+              if ((int32_t)i < ucap) (d ? ul1 : ul0)[i] = v;
+              else gstore((d ? ustage1 : ustage0) + ((int32_t)i - ucap), v);
+            };
+            if (phase == 0) {
+              // ---- the next escalated (row, direction) of the window ----
+              bool found = false;
+              while (!found && li_next < we - w0) {
+                const int dd = paired ? di_next : item_dir;
+                if (s_esc[dd][li_next] != 0) {
+                  found = true;
+                  d = dd;
+                  li = li_next;
+                }
+                if (++di_next >= ndir) {
+                  di_next = 0;
+                  ++li_next;
+                }
+              }
+              if (!found) {
+                cmd = kCmdDone;
+                break;
+              }
+              src = w0 + li;
+              chunk_end = min(n, (src / M.chunk + 1) * M.chunk);
+              const int32_t prev_d = s_ls.prev[d];
+              // t1: the last row traversed here in this direction, even with wave-path rows in
+              // between (plain items: s-1 only)
+              const int32_t t1 = (prev_d >= 0 && (paired || prev_d == src - 1)) ? prev_d : -1;
+              slot1 = GSOFA_SLOT(d, t1);
+              const bool lds_t1 = t1 >= 0 && s_ls.lds_row == t1 && s_ls.lds_dir == d;  // LDS still holds X_t1
+              const bool hold = t1 == src - 1 && s_hold[d][li] != 0;           // row s holds t1 = s-1
+              // The chain base t (some t in R_s: X_s contains X_t) and the index of its facts.
+              t = t1;
+              fi = kSavedRing;
+              chained = hold && (lds_t1 || slot1 >= 0);
+              if (kPairedAny && paired && !chained) {
+                // ---- paired: early tests of the saved rows c of this direction whose other
+                // direction is saved too. An entry u of row s with u == c, or u < c in R'_c (u
+                // reaches c through vertices < c), puts c in R_s. With a staged U list only t1
+                // qualifies (the list is t1's). ----
+                const int o = 1 - d;
+                int32_t c0 = GSOFA_SV(d, 0), c1 = GSOFA_SV(d, 1), c2 = GSOFA_SV(d, 2);
+                if (want_list) {
+                  c0 = c0 == t1 ? c0 : -1;
+                  c1 = c1 == t1 ? c1 : -1;
+                  c2 = c2 == t1 ? c2 : -1;
+                }
+                const int o0 = GSOFA_SLOT(o, c0), o1 = GSOFA_SLOT(o, c1), o2 = GSOFA_SLOT(o, c2);
+                if (o0 >= 0 || o1 >= 0 || o2 >= 0) {
+                  const uint32_t* const g0 = gslot(o, o0 < 0 ? 0 : o0);
+                  const uint32_t* const g1 = gslot(o, o1 < 0 ? 0 : o1);
+                  const uint32_t* const g2 = gslot(o, o2 < 0 ? 0 : o2);
+                  auto in_other = [&](int32_t u, int32_t c, int oc, const uint32_t* g) -> bool {
+                    return oc >= 0 && (u == c || (u >= 0 && u < c && ((gload(g + (u >> 5)) >> (u & 31)) & 1u) != 0));
+                  };
+                  const int32_t* const stash = d ? stash1 : stash0;
+                  const int32_t e = lane < kEll ? stash[li * kEll + lane] : kPad;
+                  bool h0 = in_other(e, c0, o0, g0), h1 = in_other(e, c1, o1, g1), h2 = in_other(e, c2, o2, g2);
+                  if (lane == kEll - 1 && e <= -2) {
+                    h0 = h1 = h2 = false;
+                    for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !(h0 && h1 && h2); ++p) {
+                      const int32_t u = gload(M.adj[d] + p);
+                      h0 = h0 || in_other(u, c0, o0, g0);
+                      h1 = h1 || in_other(u, c1, o1, g1);
+                      h2 = h2 || in_other(u, c2, o2, g2);
+                    }
+                  }
+                  const int hits = (__ballot(h0) != 0 ? 1 : 0) | (__ballot(h1) != 0 ? 2 : 0) | (__ballot(h2) != 0 ? 4 : 0);
+                  // of the rows found in R_s, the one with the largest R (fewest vertices left)
+                  int best = -1;
+                  uint64_t best_r = 0;
+                  for (int k = 0; k < kRing; ++k)
+                    if (((hits >> k) & 1) && (best < 0 || s_facts[d][k].r > best_r)) {
+                      best = k;
+                      best_r = s_facts[d][k].r;
+                    }
+                  if (best >= 0) {
+                    chained = true;
+                    t = GSOFA_SV(d, best);
+                    fi = best;
+                  }
+                }
+              }
+              const bool chain_lds = chained && t == t1 && lds_t1;
+              // the saved bitmap of the chain base, or of t1 for a late import
+              tslot = chained ? GSOFA_SLOT(d, t) : slot1;
+              // late import (of X_{s-1} only, when s-1 is reached)
+              has_prev = !chained && t1 == src - 1 && slot1 >= 0;
+              if (lane == 0) {
+                s_cnt[0] = 1;
+                s_cnt[1] = 0;
+                s_cnt[2] = 0;
+                s_ust = 0;
+                s_unew = 0;
+                s_wall = s_wovl = s_xe = 0;
+                s_rall = s_rovl = s_useed = s_deg0 = 0;
+                s_import = 0;
+              }
+              w_all = w_ovl = x_e = r_all = r_ovl = u_seed = u_new = deg0 = 0;
+              gap = 0;
+              phase = 1;
+              if (!chained) {
+                cmd = kCmdClear;  // a fresh bitmap holding src
+                break;
+              }
+              if (!chain_lds) {
+                cmd = kCmdCopyIn;  // X_t back from its saved copy, with the bit of src set
+                cslot = tslot;
+                break;
+              }
+              if (lane == 0) {
+                const uint32_t sb = 1u << (src & 31);
+                uint32_t old;
+                if constexpr (kHbmBitmap) {
+                  old = atomicOr(&bm[src >> 5], sb);
+                } else {
+                  old = bm[src >> 5];
+                  bm[src >> 5] = old | sb;
+                }
+                s_src_was_set = (old & sb) != 0;
+              }
+            }
+            wave_sync();  // row resets, the src bit
+            if (phase == 1) {
+              // ---- the bitmap holds X_t (or only src): queues, U list base, the gap ----
+              // The carried U list keeps stale entries up to src; they are dropped when the list
+              // is written out at the end of the row.
+              ubase = (chained && want_list) ? s_facts[d][fi].ul : 0u;
+              if (lane == 0) ql0[0] = src;
+              if (chained && t < src - 1) {
+                // X_t's vertices strictly between t and src are in R_s (reached through t's set)
+                // but their own neighbours are not covered: they start the level after the seed.
+                const int32_t wlo = (t + 1) >> 5, whi = (src - 1) >> 5;
+                int32_t pos = 0;
+                for (int32_t wb = wlo; wb <= whi; wb += 64) {
+                  const int32_t w = wb + lane;
+                  uint32_t x = 0;
+                  if (w <= whi) {
+                    x = bm_get(w);
+                    const int32_t b0 = w * 32;
+                    if (b0 <= t) x &= ~0u << (t - b0) << 1;                 // above t
+                    if (b0 + 31 >= src) x &= (1u << (src - b0)) - 1u;      // below src (src > b0 here)
+                  }
+                  const uint32_t c = __popc(x);
+                  const uint32_t incl = wave_incl_scan_u32(c);
+                  int32_t p = pos + (int32_t)(incl - c);
+                  while (x) {
+                    const int b = __ffs(x) - 1;
+                    x &= x - 1;
+                    qput(ql1, qg1, qcap, p++, w * 32 + b);
+                  }
+                  pos += __builtin_amdgcn_readlane((int)incl, 63);
+                }
+                gap = (uint32_t)pos;
+                if (lane == 0) s_cnt[1] = pos;
+              }
+              level = 0;
+              imported = false;
+              phase = 2;
+              wave_sync();  // the seed, the gap queue and its count
+            }
+            if (phase == 2) {
+              // ---- BFS levels: the leader alone while a level has at most kSoloMax vertices ----
+              const uint32_t* const gprev = gslot(d, slot1 < 0 ? 0 : slot1);  // saved X_{s-1}
+              const int32_t* const stash = (kPairedAny && d) ? stash1 : stash0;
+              while (true) {
+                const int32_t ncur = s_cnt[level % 3];
+                if (ncur == 0) {
+                  phase = 3;
+                  break;
+                }
+                if (ncur > kSoloMax) {
+                  cmd = kCmdWide;  // the workgroup takes the wide levels
+                  cslot = slot1;
+                  break;
+                }
+                const bool odd = (level & 1) != 0;
+                const int32_t* const cur_l = odd ? ql1 : ql0;
+                const int32_t* const cur_g = odd ? qg1 : qg0;
+                int32_t* const nxt_l = odd ? ql0 : ql1;
+                int32_t* const nxt_g = odd ? qg0 : qg1;
+                const bool seed = level == 0;
+                int32_t nxt_n = s_cnt[(level + 1) % 3];
+                uint32_t ust = s_ust;
+                bool imp = false;  // this lane reached src-1 with an import pending
+                // One claim per lane: m < 0 claims nothing. New entries below src go to the next
+                // level, new entries above src are U. Wave-uniform call.
+                auto claim = [&](int32_t m, bool fresh_check) {
+                  const bool valid = m >= 0;
+                  bool isnew = false;
+                  if (valid) {
+                    const uint32_t bit = 1u << (m & 31);
+                    if (!fresh_check || (bm_get(m >> 5) & bit) == 0) isnew = (atomicOr(&bm[m >> 5], bit) & bit) == 0;
+                  }
+                  if (isnew && has_prev && m == src - 1) {  // import X_{src-1} instead of expanding it
+                    imp = true;
+                    isnew = false;
+                  }
+                  const bool is_r = isnew && m < src;
+                  const bool is_u = isnew && m > src;
+                  const uint64_t mr = __ballot(is_r);
+                  if (is_r) qput(nxt_l, nxt_g, qcap, nxt_n + (int32_t)rank_below(mr), m);
+                  nxt_n += __popcll(mr);
+                  u_new += is_u;
+                  if (want_list) {
+                    const uint64_t mu = __ballot(is_u);
+                    if (is_u) uput(ubase + ust + rank_below(mu), m);
+                    ust += (uint32_t)__popcll(mu);
+                  }
+                };
+                if (ncur <= kSoloNarrow) {
+                  // narrow pass: lane = (vertex, ELL slot)
+                  const int vi = lane >> 4, sl = lane & (kEll - 1);
+                  int32_t v = -1;
+                  if (vi < ncur) v = seed ? src : qget(cur_l, cur_g, qcap, vi);
+                  int32_t m = kPad;
+                  if (v >= 0) m = seed ? stash[li * kEll + sl] : gload(M.ell[d] + (int64_t)v * kEll + sl);
+                  const bool chk = v >= 0 && !seed && has_prev;
+                  const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
+                  uint32_t dupv = (v >= 0 && sl == 0 && d == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
+                  const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
+                  if (in_prev && imported) {  // covered by the imported X_{s-1}
+                    v = -1;
+                    m = kPad;
+                    dupv = 0;
+                  }
+                  const bool tail = sl == kEll - 1 && m <= -2;
+                  int32_t ov_pos = 0, ov_end = 0;
+                  if (tail) {
+                    ov_pos = -(m + 2);
+                    ov_end = gload(M.rowptr[d] + v + 1);
+                    m = kPad;
+                  }
+                  const uint32_t valid = m >= 0 ? 1u : 0u;
+                  if (seed) {
+                    deg0 += valid + dupv;
+                    u_seed += (m > src) ? 1u : 0u;  // entries of A's row above the diagonal
+                  } else {
+                    w_all += valid + dupv;
+                    if (in_prev) w_ovl += valid + dupv;
+                    if (v >= 0 && sl == 0) {
+                      r_all += 1;
+                      if (in_prev) r_ovl += 1;
+                    }
+                  }
+                  x_e += valid;
+                  claim(m, false);
+                  // CSR tails of long rows: the wave walks each one 64 entries at a time
+                  uint64_t tm = __ballot(tail);
+                  while (tm != 0) {
+                    const int tl = __ffsll((unsigned long long)tm) - 1;
+                    tm &= tm - 1;
+                    const int32_t a = __builtin_amdgcn_readlane(ov_pos, tl);
+                    const int32_t b = __builtin_amdgcn_readlane(ov_end, tl);
+                    const bool tin = __builtin_amdgcn_readlane((int)in_prev, tl) != 0;
+                    for (int32_t p = a; p < b; p += 64) {
+                      const int32_t mm = p + lane < b ? gload(M.adj[d] + p + lane) : kPad;
+                      const uint32_t vv = mm >= 0 ? 1u : 0u;
+                      if (seed) {
+                        deg0 += vv;
+                        u_seed += (mm > src) ? 1u : 0u;
+                      } else {
+                        w_all += vv;
+                        if (tin) w_ovl += vv;
+                      }
+                      x_e += vv;
+                      claim(mm, true);
+                    }
+                  }
+                } else {
+                  // wide pass of the leader: one vertex per lane, its ELL slots 8 at a time
+                  int32_t v = lane < ncur ? qget(cur_l, cur_g, qcap, lane) : -1;
+                  const bool chk = v >= 0 && has_prev;
+                  const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
+                  RowCursor rc;
+                  load_row(M, d, v, rc);
+                  uint32_t dupv = (v >= 0 && d == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
+                  const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
+                  if (in_prev && imported) {
+                    v = -1;
+                    dupv = 0;
+#pragma unroll
+                    for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
+                    rc.ov_pos = rc.ov_end = 0;
+                  }
+                  uint32_t deg = 0;
+                  uint32_t mask_r = 0, mask_u = 0;
+#pragma unroll
+                  for (int h = 0; h < kEll; h += 8) {
+                    if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
+                    uint32_t old[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                      const int32_t m = rc.e[h + j];
+                      old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                      const int32_t m = rc.e[h + j];
+                      if (m >= 0) {
+                        ++deg;
+                        if (((old[j] >> (m & 31)) & 1u) == 0) {
+                          if (has_prev && m == src - 1) imp = true;
+                          else if (m < src) mask_r |= 1u << (h + j);
+                          else if (m > src) mask_u |= 1u << (h + j);
+                        }
+                      }
+                    }
+                  }
+                  {
+                    const uint32_t c = __popc(mask_r);
+                    const uint32_t incl = wave_incl_scan_u32(c);
+                    int32_t pos = nxt_n + (int32_t)(incl - c);
+#pragma unroll
+                    for (int j = 0; j < kEll; ++j)
+                      if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
+                    nxt_n += __builtin_amdgcn_readlane((int)incl, 63);
+                  }
+                  u_new += __popc(mask_u);
+                  if (want_list) {
+                    const uint32_t c = __popc(mask_u);
+                    const uint32_t incl = wave_incl_scan_u32(c);
+                    uint32_t pos = ubase + ust + (incl - c);
+#pragma unroll
+                    for (int j = 0; j < kEll; ++j)
+                      if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
+                    ust += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                  }
+                  // CSR tails of long rows: one entry per lane per step
+                  while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+                    int32_t m = kPad;
+                    if (rc.ov_pos < rc.ov_end) m = gload(M.adj[d] + rc.ov_pos++);
+                    deg += m >= 0;
+                    claim(m, true);
+                  }
+                  x_e += deg;
+                  if (v >= 0) {
+                    deg += dupv;
+                    w_all += deg;
+                    r_all += 1;
+                    if (in_prev) {
+                      w_ovl += deg;
+                      r_ovl += 1;
+                    }
+                  }
+                }
+                if (lane == 0) {
+                  s_cnt[(level + 2) % 3] = 0;
+                  s_cnt[(level + 1) % 3] = nxt_n;
+                  s_ust = ust;
+                }
+                wave_sync();  // the next level's queue, its count and the U list
+                ++level;
+                if (!imported && __ballot(imp) != 0) {
+                  cmd = kCmdImport;  // OR in X_{src-1} before the next level
+                  cslot = slot1;
+                  break;
+                }
+              }
+              if (cmd >= 0) break;
+            }
+            if (phase == 3) {
+              // ---- this row's totals: the leader's own sums plus the cooperative phases' ----
+              const uint64_t a_w = (uint64_t)wave_sum_u32(w_all) + s_wall;
+              const uint64_t a_wo = (uint64_t)wave_sum_u32(w_ovl) + s_wovl;
+              const uint64_t a_x = (uint64_t)wave_sum_u32(x_e) + s_xe;
+              const uint64_t a_r = (uint64_t)wave_sum_u32(r_all) + s_rall;
+              const uint64_t a_ro = (uint64_t)wave_sum_u32(r_ovl) + s_rovl;
+              const uint32_t a_us = wave_sum_u32(u_seed) + s_useed;
+              const uint32_t a_un = wave_sum_u32(u_new) + s_unew;
+              const uint64_t dg0 = (uint64_t)wave_sum_u32(deg0) + s_deg0;
+              const RowFacts base = s_facts[d][fi];
+              uint64_t wr, rr;
+              uint32_t U;
+              if (imported) {
+                wr = base.wr + base.deg + (a_w - a_wo);
+                rr = base.r + 1 + (a_r - a_ro);
+                U = a_un;
+              } else if (chained) {
+                // U_t's entries up to src (the gap vertices and src itself) are not U entries of src
+                wr = base.wr + base.deg + a_w;
+                rr = base.r + 1 + a_r;
+                U = base.u - gap - (uint32_t)s_src_was_set + a_un;
+              } else {
+                wr = a_w;
+                rr = a_r;
+                U = a_un;
+              }
+              // The U list: entries above src, compacted in place and written to the structure.
+              uint32_t kept = 0;
+              bool list_ok = false;
+              if (want_list) {
+                const uint32_t len = ubase + s_ust;
+                const int64_t off = s_lp[d][li];
+                const bool lp_ok = s_lp[d][li + 1] - off == (int64_t)U;
+                int32_t* const dst = M.pool[d] + off;
+                for (uint32_t i0 = 0; i0 < len; i0 += 64) {
+                  const uint32_t i = i0 + lane;
+                  const int32_t e = i < len ? uget(i) : -1;
+                  const bool keep = e > src;
+                  const uint64_t mk = __ballot(keep);
+                  const uint32_t pos = kept + rank_below(mk);
+                  if (keep) {
+                    if (lp_ok && pos < U) gstore(dst + pos, e);
+                    uput(pos, e);
+                  }
+                  kept += (uint32_t)__popcll(mk);
+                }
+                list_ok = lp_ok && kept == U;
+              }
+              const bool next_in = src + 1 < r1;
+              // Save the bitmap for later rows: paired, whenever a row follows in the block (it
+              // may chain from this one across wave-path rows, and the LDS copy will not survive
+              // the other direction); plain, when the next row is escalated and does not hold
+              // this one (only an import can use it).
+              const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+              const int wslot = s_ls.wp[d];  // the ring slot this row's save takes
+              if (lane == 0) {
+                const RowFacts f{wr, dg0, rr, U, want_list ? kept : 0u};
+                s_facts[d][kSavedRing] = f;
+                if (save) s_facts[d][wslot] = f;
+                if (mb != last_mat) {
+                  flush_totals(mats, last_mat, tot);
+                  last_mat = mb;
+                }
+                gstore(M.cnt[d] + src, (int32_t)U);
+                tot.fill += U - a_us;
+                tot.heavy += 1;
+                tot.x_edges += a_x;
+                tot.x_verts += a_r + 1;
+                tot.x_out += U;
+                if (d == 0) {
+                  tot.w += dg0 + wr;
+                  tot.r += rr;
+                  tot.nnz_u += U;
+                } else {
+                  tot.nnz_l += U;
+                  // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
+                  uint32_t win[4];
+                  for (int q = 0; q < 4; ++q) {
+                    const int32_t lo = src + 1 + 32 * q;  // first row of this window word
+                    uint32_t word = 0;
+                    if (lo < chunk_end) {
+                      const int32_t wi = lo >> 5, sh = lo & 31;
+                      const uint32_t a = bm_get(wi);
+                      const uint32_t b = (wi + 1 < bw) ? bm_get(wi + 1) : 0u;
+                      word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
+                      const int32_t valid = chunk_end - lo;  // rows of the chunk left
+                      if (valid < 32) word &= (1u << valid) - 1u;
+                    }
+                    win[q] = word;
+                  }
+                  reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
+                }
+                if (want_list && !list_ok) atomicOr(&M.acc[kAccOverflow], 1ull);
+              }
+              wave_sync();  // the compacted U list and the row's facts
+              phase = 4;
+              if (save) {
+                cmd = kCmdSave;
+                cslot = wslot;
+                break;
+              }
+            }
+            if (phase == 4) {
+              // ---- bookkeeping after the row (and its save) ----
+              const bool saved = paired ? src + 1 < r1
+                                        : src + 1 < r1 && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+              if (lane == 0) {
+                if (saved) {
+                  const int wslot = s_ls.wp[d];
+                  s_ls.sv[d][wslot] = src;
+                  s_ls.wp[d] = wslot + 1 == kRing ? 0 : wslot + 1;
+                }
+                s_ls.prev[d] = src;
+                s_ls.lds_row = src;
+                s_ls.lds_dir = d;
+              }
+              wave_sync();  // the item state for the next row
+              phase = 0;
+            }
+          }
+          if (phase == 2) {
+            // the row continues after the workgroup's part: keep this turn's sums in LDS
+            const uint32_t a_w = wave_sum_u32(w_all), a_wo = wave_sum_u32(w_ovl), a_x = wave_sum_u32(x_e);
+            const uint32_t a_r = wave_sum_u32(r_all), a_ro = wave_sum_u32(r_ovl), a_us = wave_sum_u32(u_seed);
+            const uint32_t a_un = wave_sum_u32(u_new), a_d0 = wave_sum_u32(deg0);
+            if (lane == 0) {
+              s_wall += a_w;
+              s_wovl += a_wo;
+              s_xe += a_x;
+              s_rall += a_r;
+              s_rovl += a_ro;
+              s_useed += a_us;
+              s_unew += a_un;
+              s_deg0 += a_d0;
+            }
+          }
+          if (lane == 0) {
+            s_ls.li_next = li_next;
+            s_ls.di_next = di_next;
+            s_ls.phase = phase;
+            s_ls.d = d;
+            s_ls.li = li;
+            s_ls.fi = fi;
+            s_ls.slot1 = slot1;
+            s_ls.tslot = tslot;
+            s_ls.level = level;
+            s_ls.src = src;
+            s_ls.t = t;
+            s_ls.chunk_end = chunk_end;
+            s_ls.chained = chained ? 1 : 0;
+            s_ls.has_prev = has_prev ? 1 : 0;
+            s_ls.imported = imported ? 1 : 0;
+            s_ls.ubase = ubase;
+            s_ls.gap = gap;
+            s_ctl.cmd = cmd;
+            s_ctl.d = d;
+            s_ctl.src = src;
+            s_ctl.level = level;
+            s_ctl.imported = imported ? 1 : 0;
+            s_ctl.slot = cslot;
+            s_ctl.ubase = ubase;
+            s_ctl.has_prev = has_prev ? 1 : 0;
+          }
+        }
+        bsync();
+        const HeavyCtl ctl = s_ctl;
+        if (ctl.cmd == kCmdDone) break;
+        const int cd = ctl.d;
+        const int32_t csrc = ctl.src;
+        if (ctl.cmd == kCmdClear) {
+          const int32_t sw = csrc >> 5;
+          const uint32_t sb = 1u << (csrc & 31);
+          for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
+          if (tid == 0) s_src_was_set = 0;
+          bsync();
+        } else if (ctl.cmd == kCmdCopyIn) {
+          // X_t back from its saved copy, with the bit of src set
+          const uint32_t* const g = gslot(cd, ctl.slot);
+          for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
+            uint32_t pw[kImportBatch];
+#pragma unroll
+            for (int q = 0; q < kImportBatch; ++q) {
+              const int32_t w = wb + q * kThreads + tid;
+              pw[q] = w < bw ? gload(g + w) : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < kImportBatch; ++q) {
+              const int32_t w = wb + q * kThreads + tid;
+              if (w < bw) {
+                uint32_t x = pw[q];
+                if (w == (csrc >> 5)) {
+                  s_src_was_set = (x >> (csrc & 31)) & 1u;
+                  x |= 1u << (csrc & 31);
+                }
+                bm[w] = x;
+              }
+            }
+          }
+          bsync();
+        } else if (ctl.cmd == kCmdSave) {
+          uint32_t* const gout = gslot(cd, ctl.slot);
+          for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm_get(w));
+          bsync();
+        } else {
+          // kCmdImport and kCmdWide share the level machinery of all waves.
+          int32_t* const ustage = cd ? ustage1 : ustage0;
+          int32_t* const ul = cd ? ul1 : ul0;
           auto uput = [&](uint32_t i, int32_t v) {
             if ((int32_t)i < ucap) ul[i] = v;
             else gstore(ustage + ((int32_t)i - ucap), v);
           };
-          const int32_t prev_d = d ? prev1 : prev0;
-          // t1: the last row traversed here in this direction, even with wave-path rows in
-          // between (plain items: s-1 only)
-          const int32_t t1 = (prev_d >= 0 && (paired || prev_d == src - 1)) ? prev_d : -1;
-          const int slot1 = GSOFA_SLOT(d, t1);
-          const bool lds_t1 = t1 >= 0 && lds_row == t1 && lds_dir == d;  // LDS still holds X_t1
-          const bool hold = t1 == src - 1 && s_hold[d][li] != 0;           // row s holds t1 = s-1
-          // The chain base t (some t in R_s: X_s contains X_t) and the index of its facts.
-          int32_t t = t1;
-          int fi = kSavedRing;
-          bool chained = hold && (lds_t1 || slot1 >= 0);
-          if (kPairedAny && paired && !chained) {
-            // ---- paired: early tests of the saved rows c of this direction whose other
-            // direction is saved too. An entry u of row s with u == c, or u < c in R'_c (u reaches
-            // c through vertices < c), puts c in R_s. With a staged U list only t1 qualifies
-            // (the list is t1's). ----
-            const int o = 1 - d;
-            int32_t c0 = GSOFA_SV(d, 0), c1 = GSOFA_SV(d, 1), c2 = GSOFA_SV(d, 2);
-            if (want_list) {
-              c0 = c0 == t1 ? c0 : -1;
-              c1 = c1 == t1 ? c1 : -1;
-              c2 = c2 == t1 ? c2 : -1;
-            }
-            const int o0 = GSOFA_SLOT(o, c0), o1 = GSOFA_SLOT(o, c1), o2 = GSOFA_SLOT(o, c2);
-            if (o0 >= 0 || o1 >= 0 || o2 >= 0) {
-              if (wave == 0) {
-                const uint32_t* const g0 = gslot(o, o0 < 0 ? 0 : o0);
-                const uint32_t* const g1 = gslot(o, o1 < 0 ? 0 : o1);
-                const uint32_t* const g2 = gslot(o, o2 < 0 ? 0 : o2);
-                auto in_other = [&](int32_t u, int32_t c, int oc, const uint32_t* g) -> bool {
-                  return oc >= 0 && (u == c || (u >= 0 && u < c && ((gload(g + (u >> 5)) >> (u & 31)) & 1u) != 0));
-                };
-                const int32_t e = lane < kEll ? gload(M.ell[d] + (int64_t)src * kEll + lane) : kPad;
-                bool h0 = in_other(e, c0, o0, g0), h1 = in_other(e, c1, o1, g1), h2 = in_other(e, c2, o2, g2);
-                if (lane == kEll - 1 && e <= -2) {
-                  h0 = h1 = h2 = false;
-                  for (int32_t p = -(e + 2), pe = gload(M.rowptr[d] + src + 1); p < pe && !(h0 && h1 && h2); ++p) {
-                    const int32_t u = gload(M.adj[d] + p);
-                    h0 = h0 || in_other(u, c0, o0, g0);
-                    h1 = h1 || in_other(u, c1, o1, g1);
-                    h2 = h2 || in_other(u, c2, o2, g2);
+          const uint32_t* const gprev = gslot(cd, ctl.slot < 0 ? 0 : ctl.slot);  // saved X_{s-1}
+          const bool chas_prev = ctl.has_prev != 0;
+          const uint32_t cubase = ctl.ubase;
+          bool cimported = ctl.imported != 0;
+          uint32_t p_w = 0, p_wo = 0, p_x = 0, p_r = 0, p_ro = 0, p_un = 0;
+          // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
+          auto import_prev = [&]() {
+            for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
+              uint32_t pw[kImportBatch];
+#pragma unroll
+              for (int q = 0; q < kImportBatch; ++q) {
+                const int32_t w = wb + q * kThreads + tid;
+                pw[q] = w < bw ? gload(gprev + w) : 0u;
+              }
+#pragma unroll
+              for (int q = 0; q < kImportBatch; ++q) {
+                const int32_t w = wb + q * kThreads + tid;
+                uint32_t nb = 0;
+                if (w < bw) {
+                  uint32_t cw;
+                  if constexpr (kHbmBitmap) {
+                    cw = pw[q] ? atomicOr(&bm[w], pw[q]) : 0u;  // keeps the level's L2 claims
+                  } else {
+                    cw = bm[w];
+                    bm[w] = cw | pw[q];
                   }
+                  nb = pw[q] & ~cw;
+                  const int32_t b0 = w * 32;
+                  if (b0 + 31 <= csrc) nb = 0;
+                  else if (b0 <= csrc) nb &= ~0u << ((csrc - b0) & 31) << 1;  // keep bits above src
                 }
-                const uint64_t b0 = __ballot(h0), b1 = __ballot(h1), b2 = __ballot(h2);
-                if (lane == 0) s_early = (b0 != 0 ? 1 : 0) | (b1 != 0 ? 2 : 0) | (b2 != 0 ? 4 : 0);
-              }
-              bsync();
-              // of the rows found in R_s, the one with the largest R (fewest vertices left)
-              const int hits = s_early;
-              int best = -1;
-              uint64_t best_r = 0;
-              for (int k = 0; k < kRing; ++k)
-                if (((hits >> k) & 1) && (best < 0 || s_facts[d][k].r > best_r)) {
-                  best = k;
-                  best_r = s_facts[d][k].r;
-                }
-              if (best >= 0) {
-                chained = true;
-                t = GSOFA_SV(d, best);
-                fi = best;
-              }
-            }
-          }
-          const bool chain_lds = chained && t == t1 && lds_t1;
-          const bool chain_copy = chained && !chain_lds;
-          // the saved bitmap of the chain base, or of t1 for a late import
-          const int tslot = chained ? GSOFA_SLOT(d, t) : slot1;
-          uint32_t* const gprev = gslot(d, tslot < 0 ? 0 : tslot);
-          // late import (of X_{s-1} only, when s-1 is reached)
-          const bool has_prev = !chained && t1 == src - 1 && slot1 >= 0;
-
-          // ---- start: bitmap, staged U list, counters ----
-          uint32_t kept = 0;  // the carried U list kept for this row (when filtered)
-          if (!chained) {
-            const int32_t sw = src >> 5;
-            const uint32_t sb = 1u << (src & 31);
-            for (int32_t w = tid; w < bw; w += kThreads) bm[w] = w == sw ? sb : 0u;
-            if (tid == 0) s_src_was_set = 0;
-          } else {
-            if (chain_copy) {
-              // X_t back from its saved copy, with the bit of src set
-              for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
-                uint32_t pw[kImportBatch];
-#pragma unroll
-                for (int q = 0; q < kImportBatch; ++q) {
-                  const int32_t w = wb + q * kThreads + tid;
-                  pw[q] = w < bw ? gload(gprev + w) : 0u;
-                }
-#pragma unroll
-                for (int q = 0; q < kImportBatch; ++q) {
-                  const int32_t w = wb + q * kThreads + tid;
-                  if (w < bw) {
-                    uint32_t x = pw[q];
-                    if (w == (src >> 5)) {
-                      s_src_was_set = (x >> (src & 31)) & 1u;
-                      x |= 1u << (src & 31);
-                    }
-                    bm[w] = x;
-                  }
-                }
-              }
-            }
-            if (want_list && t == src - 1) {
-              // every carried entry is above src-1: only src itself can drop out. The one
-              // thread that finds it moves the last entry into its place.
-              const uint32_t len = s_facts[d][fi].ul;
-              for (uint32_t i = tid; i < len; i += kThreads)
-                if (uget(i) == src) {
-                  uput(i, uget(len - 1));
-                  s_drop = 1;
-                }
-            } else if (want_list) {
-              // keep the carried U entries above src, in place (reads of a round finish before its writes)
-              const uint32_t len = s_facts[d][fi].ul;
-              for (uint32_t base = 0; base < len; base += kThreads) {
-                const uint32_t i = base + tid;
-                const int32_t m = i < len ? uget(i) : -1;
-                const bool keep = m > src;
-                const uint64_t bal = __ballot(keep);
-                if (lane == 0) s_scan[wave] = (uint32_t)__popcll(bal);
-                bsync();
-                uint32_t before = 0, all = 0;
-                for (int w = 0; w < kWaves; ++w) {
-                  if (w < wave) before += s_scan[w];
-                  all += s_scan[w];
-                }
-                if (keep) uput(kept + before + rank_below(bal), m);
-                kept += all;
-                bsync();
-              }
-            }
-            if (chain_lds && tid == 0) {
-              const uint32_t sb = 1u << (src & 31);
-              uint32_t old;
-              if constexpr (kHbmBitmap) {
-                old = atomicOr(&bm[src >> 5], sb);
-              } else {
-                old = bm[src >> 5];
-                bm[src >> 5] = old | sb;
-              }
-              s_src_was_set = (old & sb) != 0;
-            }
-          }
-          if (tid == 0) {
-            ql0[0] = src;
-            s_cnt[0] = 1;
-            s_cnt[1] = 0;
-            s_cnt[2] = 0;
-            s_gap = 0;
-            s_ust = 0;  // new U entries of this row; they go after the carried ones (ubase)
-            s_unew = 0;
-            s_useed = 0;
-            s_wall = s_wovl = s_xe = 0;
-            s_rall = s_rovl = 0;
-            s_deg0 = 0;
-            s_import = 0;
-          }
-          bsync();
-          // the carried part of this row's U list (new entries are appended after it)
-          const uint32_t ubase = !(chained && want_list) ? 0u
-                                 : (t == src - 1 ? s_facts[d][fi].ul - (uint32_t)s_drop : kept);
-          if (chained && t < src - 1) {
-            // X_t's vertices strictly between t and src are in R_s (reached through t's set) but
-            // their own neighbours are not covered: they start the first level after the seed.
-            const int32_t wlo = (t + 1) >> 5, whi = (src - 1) >> 5;
-            for (int32_t w = wlo + tid; w <= whi; w += kThreads) {
-              uint32_t x = bm_get(w);
-              const int32_t b0 = w * 32;
-              if (b0 <= t) x &= ~0u << (t - b0) << 1;                 // above t
-              if (b0 + 31 >= src) x &= (1u << (src - b0)) - 1u;      // below src (src > b0 here)
-              if (x) {
-                int32_t pos = atomicAdd(&s_cnt[1], (int32_t)__popc(x));
-                atomicAdd(&s_gap, (uint32_t)__popc(x));
-                while (x) {
-                  const int b = __ffs(x) - 1;
-                  x &= x - 1;
-                  qput(ql1, qg1, qcap, pos++, b0 + b);
-                }
-              }
-            }
-            bsync();
-          }
-
-          // this row's sums (each vertex is expanded at most once per row, so every sum stays
-          // below the input nnz < 2^31)
-          uint32_t w_all = 0, w_ovl = 0, x_e = 0;
-          uint32_t r_all = 0, r_ovl = 0, u_seed = 0, u_new = 0;
-          // Whether X_{src-1} was imported: a register, identical in every thread (each thread
-          // takes the same branches below). A shared flag written inside the import block could be
-          // seen by a slower wave before it evaluated the block's condition.
-          bool imported = false;
-          for (int level = 0;; ++level) {
-            const int32_t ncur = s_cnt[level % 3];
-            if (ncur == 0) break;
-            if (tid == 0) s_cnt[(level + 2) % 3] = 0;
-            const bool odd = (level & 1) != 0;
-            const int32_t* const cur_l = odd ? ql1 : ql0;
-            const int32_t* const cur_g = odd ? qg1 : qg0;
-            int32_t* const nxt_l = odd ? ql0 : ql1;
-            int32_t* const nxt_g = odd ? qg0 : qg1;
-            int32_t* nxt_cnt = &s_cnt[(level + 1) % 3];
-            const bool seed = level == 0;
-            for (int32_t base = 0; base < ncur; base += kThreads) {
-              const int32_t e = base + tid;
-              int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
-              // issued together: membership in the saved X_{s-1}, the row, its duplicate surplus
-              const bool chk = v >= 0 && !seed && has_prev;
-              const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
-              RowCursor rc;
-              load_row(M, d, v, rc);
-              const uint32_t dupv = (v >= 0 && d == 0 && M.dup_all) ? (uint32_t)gload(M.dup_all + v) : 0u;
-              const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
-              if (in_prev && imported) {  // covered by the imported X_{s-1}
-                v = -1;
-#pragma unroll
-                for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
-                rc.ov_pos = rc.ov_end = 0;
-              }
-              uint32_t deg = 0;
-              // Claim the bits of 8 slots at a time: the LDS atomics are independent and in flight
-              // together (measured faster than a plain read first). Rows are packed, so the second
-              // group is skipped when no lane has a ninth entry. New entries below src go to the
-              // next level, new entries above src are U.
-              uint32_t mask_r = 0, mask_u = 0;
-#pragma unroll
-              for (int h = 0; h < kEll; h += 8) {
-                if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
-                uint32_t old[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                  const int32_t m = rc.e[h + j];
-                  old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                  const int32_t m = rc.e[h + j];
-                  if (m >= 0) {
-                    ++deg;
-                    if (seed && m > src) ++u_seed;  // entries of A's row above the diagonal
-                    if (((old[j] >> (m & 31)) & 1u) == 0) {
-                      if (has_prev && m == src - 1) s_import = 1;  // import X_{src-1} instead of expanding it
-                      else if (m < src) mask_r |= 1u << (h + j);
-                      else if (m > src) mask_u |= 1u << (h + j);
-                    }
-                  }
-                }
-              }
-              // One queue reservation per wave (exclusive scan of the per-lane counts).
-              {
-                const uint32_t c = __popc(mask_r);
-                const uint32_t incl = wave_incl_scan_u32(c);
-                const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-                if (tot_w != 0) {
-                  int32_t basepos = 0;
-                  if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
-                  int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
-#pragma unroll
-                  for (int j = 0; j < kEll; ++j)
-                    if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
-                }
-              }
-              u_new += __popc(mask_u);
-              if (want_list) {
-                const uint32_t c = __popc(mask_u);
-                const uint32_t incl = wave_incl_scan_u32(c);
-                const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-                if (tot_w != 0) {
-                  uint32_t basepos = 0;
-                  if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
-                  uint32_t pos = ubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
-#pragma unroll
-                  for (int j = 0; j < kEll; ++j)
-                    if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
-                }
-              }
-              // the CSR tail of a long row, one entry per lane per step
-              auto visit = [&](int32_t m) {
-                const bool active = m >= 0;
-                deg += active;
-                if (seed && m > src) ++u_seed;
-                bool isnew = false;
-                if (active) {
-                  const uint32_t bit = 1u << (m & 31);
-                  uint32_t* wp = &bm[m >> 5];
-                  if ((bm_get(m >> 5) & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
-                }
-                if (isnew && has_prev && m == src - 1) {
-                  s_import = 1;
-                  isnew = false;
-                }
-                const bool is_r = isnew && m < src;
-                const bool is_u = isnew && m > src;
-                const uint64_t mr = __ballot(is_r);
-                if (mr) {
-                  const int leader = __ffsll((unsigned long long)mr) - 1;
-                  int32_t basepos = 0;
-                  if (lane == leader) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
-                  basepos = __shfl(basepos, leader, 64);
-                  if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
-                }
-                u_new += is_u;
+                p_un += __popc(nb);
                 if (want_list) {
-                  const uint64_t mu = __ballot(is_u);
-                  if (mu) {
-                    const int leader = __ffsll((unsigned long long)mu) - 1;
-                    uint32_t basepos = 0;
-                    if (lane == leader) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
-                    basepos = ubase + __shfl(basepos, leader, 64);
-                    if (is_u) uput(basepos + rank_below(mu), m);
+                  const uint32_t cnt = __popc(nb);
+                  // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
+                  const uint32_t incl = wave_incl_scan_u32(cnt);
+                  const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                  uint32_t basep = 0;
+                  if (lane == 0 && wave_total) basep = atomicAdd(&s_ust, wave_total);
+                  basep = cubase + (uint32_t)__builtin_amdgcn_readlane((int)basep, 0);
+                  uint32_t pos = basep + incl - cnt;
+                  while (nb) {
+                    const int b = __ffs(nb) - 1;
+                    nb &= nb - 1;
+                    uput(pos++, w * 32 + b);
                   }
-                }
-              };
-              while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
-                int32_t m = kPad;
-                if (rc.ov_pos < rc.ov_end) m = gload(M.adj[d] + rc.ov_pos++);
-                visit(m);
-              }
-              x_e += deg;
-              if (v >= 0) deg += dupv;
-              if (seed) {
-                if (tid == 0) s_deg0 = (int32_t)deg;
-              } else if (v >= 0) {
-                w_all += deg;
-                r_all += 1;
-                if (in_prev) {
-                  w_ovl += deg;
-                  r_ovl += 1;
                 }
               }
             }
+          };
+          int clevel = ctl.level;
+          if (ctl.cmd == kCmdImport) {
+            import_prev();
+            cimported = true;
             bsync();
-            // s_import only ever goes 0 -> 1 during a row, and only in visits before this barrier
-            if (!imported && s_import != 0) {
-              // OR in X_{src-1}; its new bits above src are U entries of src, the rest is R_{src-1}
-              for (int32_t wb = 0; wb < bw; wb += kThreads * kImportBatch) {
-                uint32_t pw[kImportBatch];
+          } else {
+            // levels wider than kSoloMax, all waves (one vertex per thread), until one is narrow
+            const MatDev& CM = M;
+            for (;; ++clevel) {
+              const int32_t ncur = s_cnt[clevel % 3];
+              if (ncur <= kSoloMax) break;
+              if (tid == 0) s_cnt[(clevel + 2) % 3] = 0;
+              const bool odd = (clevel & 1) != 0;
+              const int32_t* const cur_l = odd ? ql1 : ql0;
+              const int32_t* const cur_g = odd ? qg1 : qg0;
+              int32_t* const nxt_l = odd ? ql0 : ql1;
+              int32_t* const nxt_g = odd ? qg0 : qg1;
+              int32_t* nxt_cnt = &s_cnt[(clevel + 1) % 3];
+              for (int32_t base = 0; base < ncur; base += kThreads) {
+                const int32_t e = base + tid;
+                int32_t v = e < ncur ? qget(cur_l, cur_g, qcap, e) : -1;
+                // issued together: membership in the saved X_{s-1}, the row, its duplicate surplus
+                const bool chk = v >= 0 && chas_prev;
+                const uint32_t pword = chk ? gload(gprev + (v >> 5)) : 0u;
+                RowCursor rc;
+                load_row(CM, cd, v, rc);
+                const uint32_t dupv = (v >= 0 && cd == 0 && CM.dup_all) ? (uint32_t)gload(CM.dup_all + v) : 0u;
+                const bool in_prev = chk && ((pword >> (v & 31)) & 1u) != 0;
+                if (in_prev && cimported) {  // covered by the imported X_{s-1}
+                  v = -1;
 #pragma unroll
-                for (int q = 0; q < kImportBatch; ++q) {
-                  const int32_t w = wb + q * kThreads + tid;
-                  pw[q] = w < bw ? gload(gprev + w) : 0u;
+                  for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
+                  rc.ov_pos = rc.ov_end = 0;
                 }
+                uint32_t deg = 0;
+                // Claim the bits of 8 slots at a time: the LDS atomics are independent and in
+                // flight together. Rows are packed, so the second group is skipped when no lane
+                // has a ninth entry. New entries below src go to the next level, new entries above
+                // src are U.
+                uint32_t mask_r = 0, mask_u = 0;
 #pragma unroll
-                for (int q = 0; q < kImportBatch; ++q) {
-                  const int32_t w = wb + q * kThreads + tid;
-                  uint32_t nb = 0;
-                  if (w < bw) {
-                    uint32_t cw;
-                    if constexpr (kHbmBitmap) {
-                      cw = pw[q] ? atomicOr(&bm[w], pw[q]) : 0u;  // keeps the level's L2 claims
-                    } else {
-                      cw = bm[w];
-                      bm[w] = cw | pw[q];
-                    }
-                    nb = pw[q] & ~cw;
-                    const int32_t b0 = w * 32;
-                    if (b0 + 31 <= src) nb = 0;
-                    else if (b0 <= src) nb &= ~0u << ((src - b0) & 31) << 1;  // keep bits above src
+                for (int h = 0; h < kEll; h += 8) {
+                  if (h > 0 && __ballot(rc.e[h] >= 0) == 0) break;
+                  uint32_t old[8];
+#pragma unroll
+                  for (int j = 0; j < 8; ++j) {
+                    const int32_t m = rc.e[h + j];
+                    old[j] = m >= 0 ? atomicOr(&bm[m >> 5], 1u << (m & 31)) : ~0u;
                   }
-                  u_new += __popc(nb);
-                  if (want_list) {
-                    const uint32_t cnt = __popc(nb);
-                    // wave-aggregated reservation (inclusive scan of the counts), then each lane writes
-                    const uint32_t incl = wave_incl_scan_u32(cnt);
-                    const uint32_t wave_total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-                    uint32_t base = 0;
-                    if (lane == 0 && wave_total) base = atomicAdd(&s_ust, wave_total);
-                    base = ubase + (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
-                    uint32_t pos = base + incl - cnt;
-                    while (nb) {
-                      const int b = __ffs(nb) - 1;
-                      nb &= nb - 1;
-                      uput(pos++, w * 32 + b);
+#pragma unroll
+                  for (int j = 0; j < 8; ++j) {
+                    const int32_t m = rc.e[h + j];
+                    if (m >= 0) {
+                      ++deg;
+                      if (((old[j] >> (m & 31)) & 1u) == 0) {
+                        if (chas_prev && m == csrc - 1) s_import = 1;  // import X_{src-1} instead of expanding it
+                        else if (m < csrc) mask_r |= 1u << (h + j);
+                        else if (m > csrc) mask_u |= 1u << (h + j);
+                      }
                     }
+                  }
+                }
+                // One queue reservation per wave (exclusive scan of the per-lane counts).
+                {
+                  const uint32_t c = __popc(mask_r);
+                  const uint32_t incl = wave_incl_scan_u32(c);
+                  const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                  if (tot_w != 0) {
+                    int32_t basepos = 0;
+                    if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
+                    int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
+#pragma unroll
+                    for (int j = 0; j < kEll; ++j)
+                      if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
+                  }
+                }
+                p_un += __popc(mask_u);
+                if (want_list) {
+                  const uint32_t c = __popc(mask_u);
+                  const uint32_t incl = wave_incl_scan_u32(c);
+                  const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                  if (tot_w != 0) {
+                    uint32_t basepos = 0;
+                    if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
+                    uint32_t pos = cubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
+#pragma unroll
+                    for (int j = 0; j < kEll; ++j)
+                      if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
+                  }
+                }
+                // the CSR tail of a long row, one entry per lane per step
+                auto visit = [&](int32_t m) {
+                  const bool active = m >= 0;
+                  deg += active;
+                  bool isnew = false;
+                  if (active) {
+                    const uint32_t bit = 1u << (m & 31);
+                    uint32_t* wp = &bm[m >> 5];
+                    if ((bm_get(m >> 5) & bit) == 0) isnew = (atomicOr(wp, bit) & bit) == 0;
+                  }
+                  if (isnew && chas_prev && m == csrc - 1) {
+                    s_import = 1;
+                    isnew = false;
+                  }
+                  const bool is_r = isnew && m < csrc;
+                  const bool is_u = isnew && m > csrc;
+                  const uint64_t mr = __ballot(is_r);
+                  if (mr) {
+                    const int leader_lane = __ffsll((unsigned long long)mr) - 1;
+                    int32_t basepos = 0;
+                    if (lane == leader_lane) basepos = atomicAdd(nxt_cnt, (int32_t)__popcll(mr));
+                    basepos = __shfl(basepos, leader_lane, 64);
+                    if (is_r) qput(nxt_l, nxt_g, qcap, basepos + (int32_t)rank_below(mr), m);
+                  }
+                  p_un += is_u;
+                  if (want_list) {
+                    const uint64_t mu = __ballot(is_u);
+                    if (mu) {
+                      const int leader_lane = __ffsll((unsigned long long)mu) - 1;
+                      uint32_t basepos = 0;
+                      if (lane == leader_lane) basepos = atomicAdd(&s_ust, (uint32_t)__popcll(mu));
+                      basepos = cubase + __shfl(basepos, leader_lane, 64);
+                      if (is_u) uput(basepos + rank_below(mu), m);
+                    }
+                  }
+                };
+                while (__ballot(rc.ov_pos < rc.ov_end) != 0) {
+                  int32_t m = kPad;
+                  if (rc.ov_pos < rc.ov_end) m = gload(CM.adj[cd] + rc.ov_pos++);
+                  visit(m);
+                }
+                p_x += deg;
+                if (v >= 0) {
+                  deg += dupv;
+                  p_w += deg;
+                  p_r += 1;
+                  if (in_prev) {
+                    p_wo += deg;
+                    p_ro += 1;
                   }
                 }
               }
-              imported = true;
               bsync();
+              // s_import only ever goes 0 -> 1 during a row, and only in claims before this barrier
+              if (!cimported && s_import != 0) {
+                import_prev();
+                cimported = true;
+                bsync();
+              }
             }
           }
-
-          // ---- this row's totals ----
+          // this phase's sums into the row's LDS sums (per-row sums stay below the input nnz)
           {
-            const uint32_t a_w = wave_sum_u32(w_all), a_wo = wave_sum_u32(w_ovl), a_x = wave_sum_u32(x_e);
-            const uint32_t a_r = wave_sum_u32(r_all), a_ro = wave_sum_u32(r_ovl);
-            const uint32_t a_us = wave_sum_u32(u_seed), a_un = wave_sum_u32(u_new);
+            const uint32_t a_w = wave_sum_u32(p_w), a_wo = wave_sum_u32(p_wo), a_x = wave_sum_u32(p_x);
+            const uint32_t a_r = wave_sum_u32(p_r), a_ro = wave_sum_u32(p_ro), a_un = wave_sum_u32(p_un);
             if (lane == 0) {
               if (a_w) atomicAdd(&s_wall, (unsigned long long)a_w);
               if (a_wo) atomicAdd(&s_wovl, (unsigned long long)a_wo);
               if (a_x) atomicAdd(&s_xe, (unsigned long long)a_x);
-              if (a_r) atomicAdd(&s_rall, (uint32_t)a_r);
-              if (a_ro) atomicAdd(&s_rovl, (uint32_t)a_ro);
-              if (a_us) atomicAdd(&s_useed, (uint32_t)a_us);
-              if (a_un) atomicAdd(&s_unew, (uint32_t)a_un);
+              if (a_r) atomicAdd(&s_rall, a_r);
+              if (a_ro) atomicAdd(&s_rovl, a_ro);
+              if (a_un) atomicAdd(&s_unew, a_un);
             }
           }
-          // the base row's facts (of t when chained, of t1 for an import), read before thread 0
-          // writes this row's facts below
-          const RowFacts base = s_facts[d][fi];
-          bsync();
-          const uint64_t deg0 = (uint64_t)s_deg0;
-          const uint64_t wr_prev = base.wr, deg_prev = base.deg, r_prev = base.r;
-          const uint32_t u_prev = base.u;
-          uint64_t wr, rr;
-          uint32_t U;
-          if (imported) {
-            wr = wr_prev + deg_prev + (s_wall - s_wovl);
-            rr = r_prev + 1 + (uint64_t)(s_rall - s_rovl);
-            U = s_unew;
-          } else if (chained) {
-            // U_t's entries up to src (the gap vertices and src itself) are not U entries of src
-            wr = wr_prev + deg_prev + s_wall;
-            rr = r_prev + 1 + (uint64_t)s_rall;
-            U = u_prev - s_gap - (uint32_t)s_src_was_set + s_unew;
-          } else {
-            wr = s_wall;
-            rr = s_rall;
-            U = s_unew;
+          if (tid == 0) {  // the leader resumes its levels where the workgroup stopped
+            s_ls.level = clevel;
+            s_ls.imported = cimported ? 1 : 0;
           }
-          const bool next_in = src + 1 < r1;
-          // Save the bitmap for later rows: paired, whenever a row follows in the block (it may
-          // chain from this one across wave-path rows, and the LDS copy will not survive the
-          // other direction); plain, when the next row is escalated and does not hold this one
-          // (only an import can use it).
-          const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
-          const uint32_t ust_now = ubase + s_ust;  // this row's list length
-          const bool list_ok = want_list && s_lp[d][li + 1] - s_lp[d][li] == (int64_t)U && ust_now == U;
-          const int wslot = d ? wp1 : wp0;  // the ring slot this row's save takes
-          if (tid == 0) {
-            // read by later rows after the barrier that ends this row
-            const RowFacts f{wr, deg0, rr, U, ust_now};
-            s_facts[d][kSavedRing] = f;
-            if (save) s_facts[d][wslot] = f;
-            s_drop = 0;  // read into ubase by every thread before the barrier above
-            if (mb != last_mat) {
-              flush_totals(mats, last_mat, tot);
-              last_mat = mb;
-            }
-            M.cnt[d][src] = (int32_t)U;
-            tot.fill += U - s_useed;
-            tot.heavy += 1;
-            tot.x_edges += s_xe;
-            tot.x_verts += (uint64_t)s_rall + 1;
-            tot.x_out += U;
-            if (d == 0) {
-              tot.w += deg0 + wr;
-              tot.r += rr;
-              tot.nnz_u += U;
-            } else {
-              tot.nnz_l += U;
-              // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
-              uint32_t win[4];
-              for (int q = 0; q < 4; ++q) {
-                const int32_t lo = src + 1 + 32 * q;  // first row of this window word
-                uint32_t word = 0;
-                if (lo < chunk_end) {
-                  const int32_t wi = lo >> 5, sh = lo & 31;
-                  const uint32_t a = bm_get(wi);
-                  const uint32_t b = (wi + 1 < bw) ? bm_get(wi + 1) : 0u;
-                  word = sh ? ((a >> sh) | (b << (32 - sh))) : a;
-                  const int32_t valid = chunk_end - lo;  // rows of the chunk left
-                  if (valid < 32) word &= (1u << valid) - 1u;
-                }
-                win[q] = word;
-              }
-              reinterpret_cast<uint4*>(M.win)[src] = make_uint4(win[0], win[1], win[2], win[3]);
-            }
-            if (want_list && !list_ok) atomicOr(&M.acc[kAccOverflow], 1ull);
-          }
-          if (list_ok) {
-            int32_t* const dst = M.pool[d] + s_lp[d][li];
-            uint32_t i = tid;
-            for (; i + 3 * kThreads < U; i += 4 * kThreads) {
-              const int32_t a = uget(i), b = uget(i + kThreads);
-              const int32_t c = uget(i + 2 * kThreads), e = uget(i + 3 * kThreads);
-              gstore(dst + i, a);
-              gstore(dst + i + kThreads, b);
-              gstore(dst + i + 2 * kThreads, c);
-              gstore(dst + i + 3 * kThreads, e);
-            }
-            for (; i < U; i += kThreads) gstore(dst + i, uget(i));
-          }
-          if (save) {
-            uint32_t* const gout = gslot(d, wslot);
-            for (int32_t w = tid; w < bw; w += kThreads) gstore(gout + w, bm_get(w));
-            sv12 = (d && wslot == 2) ? src : sv12;
-            sv11 = (d && wslot == 1) ? src : sv11;
-            sv10 = (d && wslot == 0) ? src : sv10;
-            sv02 = (!d && wslot == 2) ? src : sv02;
-            sv01 = (!d && wslot == 1) ? src : sv01;
-            sv00 = (!d && wslot == 0) ? src : sv00;
-            const int nxt = wslot + 1 == kRing ? 0 : wslot + 1;
-            wp1 = d ? nxt : wp1;
-            wp0 = d ? wp0 : nxt;
-          }
-          if (d) prev1 = src; else prev0 = src;
-          lds_row = src;
-          lds_dir = d;
           bsync();
         }
       }
-      bsync();  // the next window's facts overwrite these
     }
   }
   if (tid == 0) flush_totals(mats, last_mat, tot);
@@ -1591,7 +1945,8 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
 }
 
 size_t heavy_lds_bytes(int32_t bw_max, int32_t qcap, bool hbm_bitmap, int32_t ucap, bool paired_any) {
-  return (hbm_bitmap ? 0 : (size_t)bw_max * 4) + (size_t)qcap * 8 + (size_t)ucap * 4 * (paired_any ? 2 : 1);
+  return (hbm_bitmap ? 0 : (size_t)bw_max * 4) + (size_t)qcap * 8 + (size_t)ucap * 4 * (paired_any ? 2 : 1) +
+         heavy_stash_bytes(paired_any);
 }
 
 // The instantiation for (threads, paired, bitmap in HBM); the HBM bitmap runs 1024-thread
