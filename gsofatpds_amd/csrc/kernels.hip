@@ -118,9 +118,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A wave-uniform value into a scalar register.
-__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-
 __device__ __forceinline__ int find_segment(const int64_t* base, int nseg, int64_t k) {
   // base[0] = 0 <= ... <= base[nseg]; returns b with base[b] <= k < base[b + 1]
   int lo = 0, hi = nseg;
@@ -882,6 +879,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ int32_t s_esc[2][kHeavyWin + 1];   // row w0 + i was escalated (per direction)
   __shared__ int32_t s_hold[2][kHeavyWin + 1];  // row w0 + i holds w0 + i - 1
   __shared__ int64_t s_lp[2][kHeavyWin + 2];    // structure offsets of rows w0 .. w0 + np
+  __shared__ unsigned long long s_escm[2];      // rows w0 .. w0 + 63 escalated, as bits (per direction)
   __shared__ HeavyCtl s_ctl;
   __shared__ LeaderState s_ls;
 
@@ -1006,6 +1004,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         }
         if (tid < np) s_esc[dd][tid] = esc_v;
         if (want_list && tid <= np) s_lp[dd][tid] = lp_v;
+        // the window's escalated rows as bits (rows w0 .. we - 1, at most 64), for the leader
+        const uint64_t em = __ballot(tid < we - w0 && esc_v != 0);
+        if (tid == 0) s_escm[dd] = em;
       }
       bsync();
 
@@ -1033,11 +1034,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               else gstore((d ? ustage1 : ustage0) + ((int32_t)i - ucap), v);
             };
             if (phase == 0) {
-              // ---- the next escalated (row, direction) of the window ----
+              // ---- the next escalated (row, direction) of the window, from the window's bits ----
+              const uint64_t em0 = s_escm[0], em1 = s_escm[1];
+              const uint64_t rows_m = paired ? (em0 | em1) : (item_dir ? em1 : em0);
+              const int nw = we - w0;
               bool found = false;
-              while (!found && li_next < we - w0) {
+              while (!found && li_next < nw) {
                 const int dd = paired ? di_next : item_dir;
-                if (s_esc[dd][li_next] != 0) {
+                if ((((dd ? em1 : em0) >> li_next) & 1ull) != 0) {
                   found = true;
                   d = dd;
                   li = li_next;
@@ -1045,6 +1049,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 if (++di_next >= ndir) {
                   di_next = 0;
                   ++li_next;
+                  // skip the rows with no escalated direction
+                  const uint64_t rest = li_next < nw ? rows_m >> li_next : 0ull;
+                  li_next = rest ? li_next + __ffsll((unsigned long long)rest) - 1 : nw;
                 }
               }
               if (!found) {
@@ -1392,14 +1399,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             }
             if (phase == 3) {
               // ---- this row's totals: the leader's own sums plus the cooperative phases' ----
-              const uint64_t a_w = (uint64_t)wave_sum_u32(w_all) + s_wall;
-              const uint64_t a_wo = (uint64_t)wave_sum_u32(w_ovl) + s_wovl;
-              const uint64_t a_x = (uint64_t)wave_sum_u32(x_e) + s_xe;
-              const uint64_t a_r = (uint64_t)wave_sum_u32(r_all) + s_rall;
-              const uint64_t a_ro = (uint64_t)wave_sum_u32(r_ovl) + s_rovl;
-              const uint32_t a_us = wave_sum_u32(u_seed) + s_useed;
-              const uint32_t a_un = wave_sum_u32(u_new) + s_unew;
-              const uint64_t dg0 = (uint64_t)wave_sum_u32(deg0) + s_deg0;
+              // (a sum no lane contributed to costs no reduction: a row that reaches nothing new
+              // below itself has only its seed sums)
+              auto wsum = [&](uint32_t x) -> uint32_t { return __ballot(x != 0) ? wave_sum_u32(x) : 0u; };
+              const uint64_t a_w = (uint64_t)wsum(w_all) + s_wall;
+              const uint64_t a_wo = (uint64_t)wsum(w_ovl) + s_wovl;
+              const uint64_t a_x = (uint64_t)wsum(x_e) + s_xe;
+              const uint64_t a_r = (uint64_t)wsum(r_all) + s_rall;
+              const uint64_t a_ro = (uint64_t)wsum(r_ovl) + s_rovl;
+              const uint32_t a_us = wsum(u_seed) + s_useed;
+              const uint32_t a_un = wsum(u_new) + s_unew;
+              const uint64_t dg0 = (uint64_t)wsum(deg0) + s_deg0;
               const RowFacts base = s_facts[d][fi];
               uint64_t wr, rr;
               uint32_t U;
