@@ -153,6 +153,8 @@ _SIGNATURES = {
                                      C.POINTER(C.c_int64)]),
     "gsofa_server_stop": (C.c_int, [C.c_void_p]),
     "gsofa_digest_fnv1a": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_uint64)]),
+    "gsofa_run_noalloc": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(Options), C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 
@@ -498,6 +500,23 @@ def supernodal_from_structure(flags, u_rowptr, u_col, l_colptr, l_row):
                                                  C.byref(nl), C.byref(nu), _i32p(xsup), p64(xl), _i32p(ls), p64(xu),
                                                  _i32p(us)))
     return xsup, xl, ls[:nl.value], xu, us[:nu.value]
+
+
+NOALLOC_TOTALS = ("fill_count", "nnz_F", "nnz_L", "nnz_U", "edge_checks", "reach", "supernodes",
+                  "supernodes_counted", "input_error")
+
+
+def run_noalloc(n: int, nnz: int, rowptr: int, adj: int, nzu: int, sup_flags: int, totals: int, stream=None,
+                **options) -> None:
+    """``gsofa_run_noalloc``: enqueue the counts-only run of one pattern without allocating device memory.
+
+    Every pointer is a device address: ``rowptr`` (n + 1 int32) and ``adj`` (nnz int32) hold rows of
+    strictly increasing column ids; ``nzu`` (n int32), ``sup_flags`` (n uint8) and ``totals``
+    (``len(NOALLOC_TOTALS)`` uint64, in that order) receive the results on ``stream``. A pattern that
+    breaks the row rule leaves ``input_error`` set and the counts of an empty pattern."""
+    o = default_options(**options)
+    _check(lib().gsofa_run_noalloc(int(n), int(nnz), rowptr, adj, C.byref(o), nzu, sup_flags, totals,
+                                   _stream_handle(stream)))
 
 
 class Batch:

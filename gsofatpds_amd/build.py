@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libgsofa.so")
 CLI = os.path.join(HERE, "gsofa_cli")
 CONVERT = os.path.join(HERE, "gsofa_convert")
 
-SOURCES = ["kernels.hip", "engine.cpp", "api.cpp", "server.cpp", "host.cpp", "gen.cpp", "supernodal.cpp",
+SOURCES = ["kernels.hip", "engine.cpp", "api.cpp", "server.cpp", "host.cpp", "gen.cpp", "supernodal.cpp", "noalloc.hip",
            "convert.cpp"]
 # -packed-fp32-ops: no v_pk_*_f32 generation on this pool (the engine has no fp32 math anyway).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",

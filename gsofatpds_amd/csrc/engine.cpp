@@ -130,6 +130,86 @@ static bool place_bitmaps_in_hbm(int32_t n_max, const Options& opt, const Device
          bitmap_bytes + kHeavyLdsReserve + heavy_stash_bytes(true) + 256 * 8 > info.lds_per_block;
 }
 
+int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Options& opt, const DeviceInfo& info,
+                     int64_t max_heavy_grid, LaunchConfig& out) {
+  out = LaunchConfig();
+  out.bw_max = (n_max + 31) / 32;
+  // static LDS and the window's rows (per direction) besides the bitmap, queues and U lists
+  const size_t reserve = kHeavyLdsReserve + heavy_stash_bytes(paired_any);
+  // Bitmaps in HBM when asked for, or when one does not fit in a workgroup's LDS next to the
+  // smallest queues (n above about 1.2 M rows): slower claims (L2 atomics), no size limit.
+  out.hbm_bitmap = place_bitmaps_in_hbm(n_max, opt, info);
+  const size_t bitmap_bytes = out.hbm_bitmap ? 0 : (size_t)out.bw_max * 4;
+  // Workgroup size of the bitmap path. Registers allow 16 waves per CU whatever the size, so the
+  // choice trades rows in flight per CU (more, smaller workgroups) against frontier width per
+  // pass. Measured: unstructured sets (paired items) and small patterns run fastest as four
+  // 256-thread workgroups per CU, structured ones as two of 512 threads up to 150,000 rows,
+  // larger ones (whose bitmap leaves LDS for one) as one of 1024.
+  auto lds_fits = [&](int wgs) { return (size_t)wgs * (bitmap_bytes + reserve + 256 * 8) <= info.lds_per_cu; };
+  int threads;
+  if (opt.heavy_threads) threads = opt.heavy_threads;
+  else if (out.hbm_bitmap) threads = 1024;
+  else if ((paired_any || n_max <= 16384) && lds_fits(4)) threads = 256;
+  else if (n_max <= 150000 && lds_fits(2)) threads = 512;
+  else threads = 1024;
+  if (threads != 256 && threads != 512 && threads != 1024)
+    return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
+  if (out.hbm_bitmap && threads != 1024)
+    return fail(GSOFA_ERR_CAPACITY, "bitmaps in HBM run 1024-thread workgroups only");
+  out.heavy_threads = threads;
+  const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : 1024 / threads;
+  const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
+  // U lists (explicit structure only): their first 4,096 (or 1,024) entries per direction in LDS
+  // when the frontier queues keep at least 2,048 entries per level; the rest stays in HBM.
+  const int64_t rem = budget > bitmap_bytes + reserve ? (int64_t)(budget - bitmap_bytes - reserve) : 0;
+  const int64_t udirs = paired_any ? 2 : 1;
+  if (opt.structure) {
+    const int64_t qmin = opt.qcap ? opt.qcap : 2048;
+    for (int32_t c : {4096, 1024})
+      if (!out.ucap && rem >= 8 * qmin + 4 * udirs * c) out.ucap = c;
+  }
+  if (opt.qcap) {
+    out.qcap = opt.qcap;
+  } else {
+    int64_t q = rem > 4 * udirs * out.ucap ? (rem - 4 * udirs * out.ucap) / 8 : 256;
+    q = std::max<int64_t>(256, std::min<int64_t>(16384, q));
+    out.qcap = (int32_t)(q & ~int64_t(63));
+  }
+  const size_t lds = heavy_lds_bytes(out.bw_max, out.qcap, out.hbm_bitmap, out.ucap, paired_any);
+  // lds holds the window's rows already; the static part is the rest
+  if (lds + kHeavyLdsReserve > info.lds_per_block)
+    return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
+  const hipError_t e = heavy_set_lds_limit(threads, lds, out.hbm_bitmap);
+  if (e != hipSuccess) return fail(GSOFA_ERR_HIP, std::string("hipFuncSetAttribute(k_heavy): ") + hipGetErrorString(e));
+  const int occ = heavy_occupancy(threads, lds, out.hbm_bitmap);
+  if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
+  const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
+  out.heavy_grid = cus * occ;
+  if (max_heavy_grid > 0 && out.heavy_grid > max_heavy_grid) out.heavy_grid = (int)max_heavy_grid;
+  out.heavy_wgs_per_cu = occ;
+  out.heavy_lds = lds;
+  // Rows per workgroup-path item: long blocks chain more, short blocks keep every workgroup
+  // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
+  // (Measured on the unstructured batch members: paired items, which carry both directions,
+  // do best with the same rule; one block per workgroup left the tail to the costliest ones.)
+  // Capped at 64 rows, or 128 with one workgroup per CU or paired items: measured best at 128 on
+  // the 659,033-row stencil (1024 x 1), at 64 on the 110,592-row one (512 x 2, 128 rows left
+  // too few blocks), at 80 = n / (2 grid) on a 165,532-row random member.
+  out.chain_block = opt.chain_block;
+  if (out.chain_block == 0) {
+    const int64_t cap = (occ == 1 || paired_any) ? 128 : 64;
+    int64_t k = (int64_t)n_max / std::max<int64_t>(1, 2 * (int64_t)out.heavy_grid);
+    out.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(cap, k));
+  }
+  const int occ_l = light_occupancy();
+  if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
+  out.light_grid = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
+  const int64_t waves = (int64_t)out.light_grid * kLightWaves;
+  out.light_block = kLightBlockMin;
+  while (out.light_block < kLightBlockMax && 2 * nrows / (2 * out.light_block) >= 4 * waves) out.light_block *= 2;
+  return GSOFA_OK;
+}
+
 int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream) {
   opt_ = opt;
   alloc_stream_ = alloc_stream;
@@ -165,82 +245,23 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (nnz_ > (int64_t)INT32_MAX - 64) return fail(GSOFA_ERR_CAPACITY, "total nnz of a set must stay below 2^31 - 64");
 
   // ---- launch configuration ----
-  bw_max_ = (n_max_ + 31) / 32;
-  // static LDS and the window's rows (per direction) besides the bitmap, queues and U lists
-  const size_t reserve = kHeavyLdsReserve + heavy_stash_bytes(paired_any_ != 0);
-  // Bitmaps in HBM when asked for, or when one does not fit in a workgroup's LDS next to the
-  // smallest queues (n above about 1.2 M rows): slower claims (L2 atomics), no size limit.
-  hbm_bitmap_ = place_bitmaps_in_hbm(n_max_, opt, info);
-  const size_t bitmap_bytes = hbm_bitmap_ ? 0 : (size_t)bw_max_ * 4;
-  // Workgroup size of the bitmap path. Registers allow 16 waves per CU whatever the size, so the
-  // choice trades rows in flight per CU (more, smaller workgroups) against frontier width per
-  // pass. Measured: unstructured sets (paired items) and small patterns run fastest as four
-  // 256-thread workgroups per CU, structured ones as two of 512 threads up to 150,000 rows,
-  // larger ones (whose bitmap leaves LDS for one) as one of 1024.
-  auto lds_fits = [&](int wgs) { return (size_t)wgs * (bitmap_bytes + reserve + 256 * 8) <= info.lds_per_cu; };
-  if (opt.heavy_threads) heavy_threads_ = opt.heavy_threads;
-  else if (hbm_bitmap_) heavy_threads_ = 1024;
-  else if ((paired_any_ || n_max_ <= 16384) && lds_fits(4)) heavy_threads_ = 256;
-  else if (n_max_ <= 150000 && lds_fits(2)) heavy_threads_ = 512;
-  else heavy_threads_ = 1024;
-  if (heavy_threads_ != 256 && heavy_threads_ != 512 && heavy_threads_ != 1024)
-    return fail(GSOFA_ERR_INVALID_ARG, "heavy_threads must be 256, 512 or 1024");
-  if (hbm_bitmap_ && heavy_threads_ != 1024)
-    return fail(GSOFA_ERR_CAPACITY, "bitmaps in HBM run 1024-thread workgroups only");
-  const int target_wgs = opt.heavy_wgs_per_cu ? opt.heavy_wgs_per_cu : 1024 / heavy_threads_;
-  const size_t budget = std::min(info.lds_per_block, info.lds_per_cu / (size_t)std::max(target_wgs, 1));
-  // U lists (explicit structure only): their first 4,096 (or 1,024) entries per direction in LDS
-  // when the frontier queues keep at least 2,048 entries per level; the rest stays in HBM.
-  const int64_t rem = budget > bitmap_bytes + reserve ? (int64_t)(budget - bitmap_bytes - reserve) : 0;
-  const int64_t udirs = paired_any_ ? 2 : 1;
-  ucap_ = 0;
-  if (opt.structure) {
-    const int64_t qmin = opt.qcap ? opt.qcap : 2048;
-    for (int32_t c : {4096, 1024})
-      if (!ucap_ && rem >= 8 * qmin + 4 * udirs * c) ucap_ = c;
-  }
-  if (opt.qcap) {
-    qcap_ = opt.qcap;
-  } else {
-    int64_t q = rem > 4 * udirs * ucap_ ? (rem - 4 * udirs * ucap_) / 8 : 256;
-    q = std::max<int64_t>(256, std::min<int64_t>(16384, q));
-    qcap_ = (int32_t)(q & ~int64_t(63));
-  }
-  const size_t lds = heavy_lds_bytes(bw_max_, qcap_, hbm_bitmap_, ucap_, paired_any_ != 0);
-  // lds holds the window's rows already; the static part is the rest
-  if (lds + kHeavyLdsReserve > info.lds_per_block)
-    return fail(GSOFA_ERR_CAPACITY, "heavy LDS request exceeds the device limit");
-  GSOFA_HIP(heavy_set_lds_limit(heavy_threads_, lds, hbm_bitmap_), "hipFuncSetAttribute(k_heavy)");
-  const int occ = heavy_occupancy(heavy_threads_, lds, hbm_bitmap_);
-  if (occ <= 0) return fail(GSOFA_ERR_CAPACITY, "k_heavy cannot be resident with this LDS size");
-  const int cus = opt.cu_share > 0 ? std::min(opt.cu_share, info.cus) : info.cus;
-  heavy_grid_ = cus * occ;
-  heavy_wgs_per_cu_ = occ;
-  // Rows per workgroup-path item: long blocks chain more, short blocks keep every workgroup
-  // busy; aim for about two blocks per resident workgroup and direction on the largest matrix.
-  // (Measured on the unstructured batch members: paired items, which carry both directions,
-  // do best with the same rule; one block per workgroup left the tail to the costliest ones.)
-  // Capped at 64 rows, or 128 with one workgroup per CU or paired items: measured best at 128 on
-  // the 659,033-row stencil (1024 x 1), at 64 on the 110,592-row one (512 x 2, 128 rows left
-  // too few blocks), at 80 = n / (2 grid) on a 165,532-row random member.
-  if (opt_.chain_block == 0) {
-    const int64_t cap = (heavy_wgs_per_cu_ == 1 || paired_any_) ? 128 : 64;
-    int64_t k = (int64_t)n_max_ / std::max<int64_t>(1, 2 * (int64_t)heavy_grid_);
-    opt_.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(cap, k));
-  }
+  LaunchConfig cfg;
+  GSOFA_TRY(configure_launch(n_max_, nrows_, paired_any_ != 0, opt, info, 0, cfg));
+  bw_max_ = cfg.bw_max;
+  hbm_bitmap_ = cfg.hbm_bitmap;
+  heavy_threads_ = cfg.heavy_threads;
+  heavy_grid_ = cfg.heavy_grid;
+  heavy_wgs_per_cu_ = cfg.heavy_wgs_per_cu;
+  light_grid_ = cfg.light_grid;
+  light_block_ = cfg.light_block;
+  qcap_ = cfg.qcap;
+  ucap_ = cfg.ucap;
+  opt_.chain_block = cfg.chain_block;
   blkq_base_h_.assign(nmats_ + 1, 0);
   for (int b = 0; b < nmats_; ++b)
     blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt_.chain_block - 1) / opt_.chain_block;
   nblk_total_ = blkq_base_h_[nmats_];
   nb_max_ = (int32_t)(((int64_t)n_max_ + opt_.chain_block - 1) / opt_.chain_block);
-  const int occ_l = light_occupancy();
-  if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
-  light_grid_ = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
-  {
-    const int64_t waves = (int64_t)light_grid_ * kLightWaves;
-    light_block_ = kLightBlockMin;
-    while (light_block_ < kLightBlockMax && 2 * nrows_ / (2 * light_block_) >= 4 * waves) light_block_ *= 2;
-  }
   for (int b = 0; b < nmats_; ++b)
     item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)ns_[b] + light_block_ - 1) / light_block_);
   nitems_ = item_base_h_[nmats_];

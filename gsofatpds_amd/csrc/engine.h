@@ -61,6 +61,20 @@ struct DeviceInfo {
 };
 int device_info(int device, DeviceInfo& info);
 
+// Launch configuration of a set: bitmap-path workgroup shape, its LDS split, the grids and the
+// block sizes of both paths.
+struct LaunchConfig {
+  int heavy_threads = 1024, heavy_grid = 0, heavy_wgs_per_cu = 0, light_grid = 0;
+  int32_t qcap = 0, ucap = 0, bw_max = 0, chain_block = 0, light_block = kLightBlockMin;
+  bool hbm_bitmap = false;
+  size_t heavy_lds = 0;  // dynamic LDS bytes of k_heavy
+};
+// Picks the configuration for a set whose largest matrix has n_max rows (nrows in all) and sets
+// k_heavy's LDS limit. max_heavy_grid > 0 caps the bitmap-path workgroups (each holds a spill
+// region of heavy_region_ints(n_max) ints), for callers whose regions live in a fixed buffer.
+int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Options& opt, const DeviceInfo& info,
+                     int64_t max_heavy_grid, LaunchConfig& out);
+
 // Device state for one matrix (plain plan) or many (batch); all matrices share one launch of
 // each kernel.
 class DeviceSet {

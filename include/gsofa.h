@@ -288,6 +288,24 @@ int gsofa_server_stats(gsofa_server* srv, int64_t* completed, int32_t* running, 
 /* Drops pending requests, waits for running ones, releases everything. */
 int gsofa_server_stop(gsofa_server* srv);
 
+/* ---- allocation-free run ------------------------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The counts-only run of
+ * gsofa_plan_create + gsofa_run + gsofa_result_get / gsofa_nzu_get / gsofa_supernodes_get, enqueued
+ * on `hip_stream` with no allocation: every buffer beyond the inputs and outputs lives in one
+ * static 128 MiB device array per GPU, held until the process ends. Limits: n <= 65,536 rows and
+ * nnz <= 1,048,576 entries (GSOFA_ERR_CAPACITY beyond them); structure, autotune and
+ * bitmap_in_hbm = 1 are rejected; pair_directions -1 means off. All pointers are device addresses:
+ * rowptr (n + 1) and adj (nnz) with rowptr[0] = 0, rowptr[n] = nnz and rows of strictly increasing
+ * column ids in [0, n); outputs nzu (n), sup_flags (n) and totals (GSOFA_NOALLOC_TOTALS: fill_count,
+ * nnz_F, nnz_L, nnz_U, edge_checks, reach, supernodes, supernodes_counted, input_error). A pattern
+ * that breaks the row rule is checked on the device: input_error is 1 and the counts are those of
+ * an empty pattern. Calls on one GPU share its array: calls that overlap on one GPU (two streams,
+ * two threads) race. */
+#define GSOFA_NOALLOC_TOTALS 9
+int gsofa_run_noalloc(int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* adj, const gsofa_options* opt,
+                      int32_t* nzu, uint8_t* sup_flags, uint64_t* totals, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif
