@@ -210,6 +210,43 @@ int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Option
   return GSOFA_OK;
 }
 
+int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
+  auto hf = [](hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? GSOFA_ERR_OUT_OF_MEMORY : GSOFA_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+  };
+#define GSOFA_SEQ(x, what)                      \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) return hf(e_, what);  \
+  } while (0)
+  if (ev) GSOFA_SEQ(hipEventRecord(ev[0], s), "hipEventRecord");
+  GSOFA_SEQ(launch_fill_u64(q.acc, (int64_t)q.nmats * kAccSlots, 0ull, s), "clear accumulators");
+  GSOFA_SEQ(launch_fill_u64(reinterpret_cast<unsigned long long*>(q.ws), 4, 0ull, s), "clear work state");
+  GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.esc), q.esc_words, 0, s), "clear escalation flags");
+  GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.blk_flag), 2 * q.nblk_total, 0, s), "clear block flags");
+  GSOFA_SEQ(launch_ell_build(q.nrows, q.rowptr_a, q.adj_a, q.ell_a, s), "ell A");
+  GSOFA_SEQ(launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor,
+                             q.adj_t, q.block_sums, q.total, s),
+            "transpose");
+  GSOFA_SEQ(launch_ell_build(q.nrows, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
+  if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
+  GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, s), "k_light");
+  // cursor, block_sums and total are free again once the transpose is built
+  GSOFA_SEQ(launch_heavy_order(q.mats, q.nblk_total, q.nmats, q.blkq_base, q.blk_flag, q.nb_max, q.bucket, q.cursor,
+                               q.block_sums, q.total, q.ws, q.heavy_items, s),
+            "workgroup-path queue");
+  if (ev) GSOFA_SEQ(hipEventRecord(ev[2], s), "hipEventRecord");
+  GSOFA_SEQ(launch_heavy(q.cfg.heavy_threads, q.cfg.heavy_grid, q.mats, q.ws, q.heavy_items, q.cfg.bw_max, q.cfg.qcap,
+                         q.n_max, q.paired_any, q.spill, q.hbm_bm, q.cfg.ucap, s),
+            "k_heavy");
+  if (ev) GSOFA_SEQ(hipEventRecord(ev[3], s), "hipEventRecord");
+  GSOFA_SEQ(launch_supernodes(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_supernodes");
+  if (ev) GSOFA_SEQ(hipEventRecord(ev[4], s), "hipEventRecord");
+#undef GSOFA_SEQ
+  return GSOFA_OK;
+}
+
 int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& opt, hipStream_t alloc_stream) {
   opt_ = opt;
   alloc_stream_ = alloc_stream;
@@ -247,6 +284,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // ---- launch configuration ----
   LaunchConfig cfg;
   GSOFA_TRY(configure_launch(n_max_, nrows_, paired_any_ != 0, opt, info, 0, cfg));
+  cfg_ = cfg;
   bw_max_ = cfg.bw_max;
   hbm_bitmap_ = cfg.hbm_bitmap;
   heavy_threads_ = cfg.heavy_threads;
@@ -408,33 +446,42 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
 }
 
 int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
-  // ev (optional, 5 events): before preparation, after preparation (ELL + transpose), after
-  // k_light, after k_heavy, after k_supernodes.
   GSOFA_HIP(hipSetDevice(opt_.device), "hipSetDevice");
-  if (ev) GSOFA_HIP(hipEventRecord(ev[0], s), "hipEventRecord");
-  GSOFA_HIP(launch_fill_u64(acc_, (int64_t)nmats_ * kAccSlots, 0ull, s), "clear accumulators");
-  GSOFA_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(ws_), 4, 0ull, s), "clear work state");
-  GSOFA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(esc_), esc_words_, 0, s), "clear escalation flags");
-  GSOFA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(blk_flag_), 2 * nblk_total_, 0, s), "clear block flags");
-  GSOFA_HIP(launch_ell_build(nrows_, rowptr_a_, adj_a_, ell_a_, s), "ell A");
-  GSOFA_HIP(launch_transpose(nrows_, nmats_, row_base_d_, rowptr_a_, adj_a_, col_cnt_, rowptr_t_, cursor_, adj_t_,
-                             block_sums_, total_, s),
-            "transpose");
-  GSOFA_HIP(launch_ell_build(nrows_, rowptr_t_, adj_t_, ell_t_, s), "ell A^T");
-  if (ev) GSOFA_HIP(hipEventRecord(ev[1], s), "hipEventRecord");
-  GSOFA_HIP(launch_light(mats_d_, nmats_, item_base_d_, nitems_, light_grid_, s), "k_light");
-  // cursor_, block_sums_ and total_ are free again once the transpose is built
-  GSOFA_HIP(launch_heavy_order(mats_d_, nblk_total_, nmats_, blkq_base_d_, blk_flag_, nb_max_, bucket_, cursor_, block_sums_,
-                               total_, ws_, heavy_items_, s),
-            "workgroup-path queue");
-  if (ev) GSOFA_HIP(hipEventRecord(ev[2], s), "hipEventRecord");
-  GSOFA_HIP(launch_heavy(heavy_threads_, heavy_grid_, mats_d_, ws_, heavy_items_, bw_max_, qcap_, n_max_, paired_any_,
-                         spill_, hbm_bm_, ucap_, s),
-            "k_heavy");
-  if (ev) GSOFA_HIP(hipEventRecord(ev[3], s), "hipEventRecord");
-  GSOFA_HIP(launch_supernodes(mats_d_, nmats_, blk_base_d_, nblocks_sup_, s), "k_supernodes");
-  if (ev) GSOFA_HIP(hipEventRecord(ev[4], s), "hipEventRecord");
-  return GSOFA_OK;
+  Sequence q;
+  q.nrows = nrows_;
+  q.nitems = nitems_;
+  q.nblk_total = nblk_total_;
+  q.nblocks_sup = nblocks_sup_;
+  q.esc_words = esc_words_;
+  q.nmats = nmats_;
+  q.nb_max = nb_max_;
+  q.n_max = n_max_;
+  q.paired_any = paired_any_;
+  q.cfg = cfg_;
+  q.rowptr_a = rowptr_a_;
+  q.adj_a = adj_a_;
+  q.ell_a = ell_a_;
+  q.col_cnt = col_cnt_;
+  q.rowptr_t = rowptr_t_;
+  q.cursor = cursor_;
+  q.adj_t = adj_t_;
+  q.ell_t = ell_t_;
+  q.block_sums = block_sums_;
+  q.total = total_;
+  q.bucket = bucket_;
+  q.spill = spill_;
+  q.acc = acc_;
+  q.ws = ws_;
+  q.heavy_items = heavy_items_;
+  q.esc = esc_;
+  q.blk_flag = blk_flag_;
+  q.hbm_bm = hbm_bm_;
+  q.mats = mats_d_;
+  q.item_base = item_base_d_;
+  q.row_base = row_base_d_;
+  q.blk_base = blk_base_d_;
+  q.blkq_base = blkq_base_d_;
+  return enqueue_sequence(q, s, ev);
 }
 
 int DeviceSet::read_results(std::vector<MatResult>& out) {

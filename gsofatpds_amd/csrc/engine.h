@@ -75,6 +75,30 @@ struct LaunchConfig {
 int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Options& opt, const DeviceInfo& info,
                      int64_t max_heavy_grid, LaunchConfig& out);
 
+// The device buffers and sizes of one launch sequence: a device set's, or the allocation-free
+// entry's (which carves the same buffers from a static array).
+struct Sequence {
+  int64_t nrows = 0, nitems = 0, nblk_total = 0, nblocks_sup = 0, esc_words = 0;
+  int nmats = 0;
+  int32_t nb_max = 0, n_max = 0;
+  int32_t paired_any = 0;
+  LaunchConfig cfg;
+  int32_t *rowptr_a = nullptr, *adj_a = nullptr, *ell_a = nullptr, *col_cnt = nullptr, *rowptr_t = nullptr;
+  int32_t *cursor = nullptr, *adj_t = nullptr, *ell_t = nullptr, *block_sums = nullptr, *total = nullptr;
+  int32_t *bucket = nullptr, *spill = nullptr;
+  unsigned long long* acc = nullptr;
+  WorkState* ws = nullptr;
+  uint64_t* heavy_items = nullptr;
+  uint8_t* esc = nullptr;
+  uint32_t *blk_flag = nullptr, *hbm_bm = nullptr;
+  MatDev* mats = nullptr;
+  int64_t *item_base = nullptr, *row_base = nullptr, *blk_base = nullptr, *blkq_base = nullptr;
+};
+// Enqueues the whole run on `s` (kernels only: no allocation, no memset node, no host
+// synchronisation). ev (optional, 5 events): before preparation, after preparation, after the
+// wave path and the workgroup-path queue, after the workgroup path, after supernodes.
+int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev);
+
 // Device state for one matrix (plain plan) or many (batch); all matrices share one launch of
 // each kernel.
 class DeviceSet {
@@ -138,6 +162,7 @@ class DeviceSet {
   int32_t* spill_ = nullptr;
   uint8_t* esc_ = nullptr;        // 2 * nrows escalation flags (padded to a multiple of 4)
   uint32_t* blk_flag_ = nullptr;  // 2 * nblk_total
+  LaunchConfig cfg_;
   int64_t nblk_total_ = 0, esc_words_ = 0;
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
   int64_t* blkq_base_d_ = nullptr;

@@ -302,23 +302,42 @@ int gsofa_run_noalloc(int32_t n, int64_t nnz, const int32_t* rowptr, const int32
   hipLaunchKernelGGL(k_noalloc_tables, dim3(1), dim3(64), 0, s, t, L.mats, L.bases);
   NA_HIP(hipGetLastError(), "input kernels");
   // the launch sequence of a plan, without its allocation
-  NA_HIP(launch_fill_u64(L.acc, kAccSlots, 0ull, s), "clear accumulators");
-  NA_HIP(launch_fill_u64(reinterpret_cast<unsigned long long*>(L.ws), 4, 0ull, s), "clear work state");
-  NA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(L.esc), esc_words, 0, s), "clear escalation flags");
-  NA_HIP(launch_fill_i32(reinterpret_cast<int32_t*>(L.blk_flag), 2 * nblk, 0, s), "clear block flags");
-  NA_HIP(launch_ell_build(n, L.rowptr_a, L.adj_a, L.ell_a, s), "ell A");
-  NA_HIP(launch_transpose(n, 1, row_base, L.rowptr_a, L.adj_a, L.col_cnt, L.rowptr_t, L.cursor, L.adj_t,
-                          L.block_sums, L.total, s),
-         "transpose");
-  NA_HIP(launch_ell_build(n, L.rowptr_t, L.adj_t, L.ell_t, s), "ell A^T");
-  NA_HIP(launch_light(L.mats, 1, item_base, nitems, cfg.light_grid, s), "k_light");
-  NA_HIP(launch_heavy_order(L.mats, nblk, 1, blkq_base, L.blk_flag, nb_max, L.bucket, L.cursor, L.block_sums, L.total,
-                            L.ws, L.heavy_items, s),
-         "workgroup-path queue");
-  NA_HIP(launch_heavy(cfg.heavy_threads, cfg.heavy_grid, L.mats, L.ws, L.heavy_items, cfg.bw_max, cfg.qcap, n,
-                      paired ? 1 : 0, L.spill, nullptr, 0, s),
-         "k_heavy");
-  NA_HIP(launch_supernodes(L.mats, 1, blk_base, nblocks_sup, s), "k_supernodes");
+  Sequence q;
+  q.nrows = n;
+  q.nitems = nitems;
+  q.nblk_total = nblk;
+  q.nblocks_sup = nblocks_sup;
+  q.esc_words = esc_words;
+  q.nmats = 1;
+  q.nb_max = nb_max;
+  q.n_max = n;
+  q.paired_any = paired ? 1 : 0;
+  q.cfg = cfg;
+  q.rowptr_a = L.rowptr_a;
+  q.adj_a = L.adj_a;
+  q.ell_a = L.ell_a;
+  q.col_cnt = L.col_cnt;
+  q.rowptr_t = L.rowptr_t;
+  q.cursor = L.cursor;
+  q.adj_t = L.adj_t;
+  q.ell_t = L.ell_t;
+  q.block_sums = L.block_sums;
+  q.total = L.total;
+  q.bucket = L.bucket;
+  q.spill = L.spill;
+  q.acc = L.acc;
+  q.ws = L.ws;
+  q.heavy_items = L.heavy_items;
+  q.esc = L.esc;
+  q.blk_flag = L.blk_flag;
+  q.hbm_bm = nullptr;
+  q.mats = L.mats;
+  q.item_base = item_base;
+  q.row_base = row_base;
+  q.blk_base = blk_base;
+  q.blkq_base = blkq_base;
+  rc = enqueue_sequence(q, s, nullptr);
+  if (rc != GSOFA_OK) return rc;
   hipLaunchKernelGGL(k_noalloc_totals, dim3(1), dim3(64), 0, s, L.acc, L.bad, n, o.n_processes, totals);
   NA_HIP(hipGetLastError(), "k_noalloc_totals");
   return GSOFA_OK;
