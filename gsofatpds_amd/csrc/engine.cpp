@@ -225,11 +225,11 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
   GSOFA_SEQ(launch_fill_u64(reinterpret_cast<unsigned long long*>(q.ws), 4, 0ull, s), "clear work state");
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.esc), q.esc_words, 0, s), "clear escalation flags");
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.blk_flag), 2 * q.nblk_total, 0, s), "clear block flags");
-  GSOFA_SEQ(launch_ell_build(q.nrows, q.rowptr_a, q.adj_a, q.ell_a, s), "ell A");
+  GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.ell_a, s), "ell A");
   GSOFA_SEQ(launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor,
                              q.adj_t, q.block_sums, q.total, s),
             "transpose");
-  GSOFA_SEQ(launch_ell_build(q.nrows, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
+  GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
   GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built

@@ -11,6 +11,10 @@ namespace gsofa {
 // read from the CSR adjacency up to rowptr[v + 1].
 constexpr int kEll = 16;
 constexpr int32_t kPad = -1;
+// A row of at most 15 entries that holds its diagonal keeps the diagonal out of its slots (it is
+// never new to a traversal) and marks it with kDiag after the off-diagonal entries: claims stop
+// at the packed entries, degree counts add the mark. Slot 15 never holds kDiag.
+constexpr int32_t kDiag = -2;
 
 // Accumulator slots (u64) per matrix.
 enum : int {
@@ -82,7 +86,8 @@ __host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint
 namespace gsofa {
 hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s);
-hipError_t launch_ell_build(int64_t nrows, const int32_t* rowptr, const int32_t* adj, int32_t* ell, hipStream_t s);
+hipError_t launch_ell_build(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
+                            const int32_t* adj, int32_t* ell, hipStream_t s);
 // Builds A^T of every matrix: col_cnt/cursor/rowptr_t have nrows (+1) entries, block_sums
 // ceil(nrows / 4096), total one int.
 hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,

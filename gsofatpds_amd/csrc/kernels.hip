@@ -146,7 +146,8 @@ __global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
 // layout: ELL-16 packing and the transpose
 // ------------------------------------------------------------------------------------------
 // Row g of the concatenated pattern: entries adj[rowptr[g] .. rowptr[g+1]) (local column ids).
-__global__ void k_ell_build(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ adj,
+__global__ void k_ell_build(int64_t nrows, int nmats, const int64_t* __restrict__ row_base,
+                            const int32_t* __restrict__ rowptr, const int32_t* __restrict__ adj,
                             int32_t* __restrict__ ell) {
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nrows; g += (int64_t)gridDim.x * blockDim.x) {
     const int32_t b = rowptr[g], e = rowptr[g + 1];
@@ -154,8 +155,19 @@ __global__ void k_ell_build(int64_t nrows, const int32_t* __restrict__ rowptr, c
     int32_t slot[kEll];
 #pragma unroll
     for (int k = 0; k < kEll; ++k) slot[k] = kPad;
-    if (deg <= kEll) {
-      for (int k = 0; k < deg; ++k) slot[k] = adj[b + k];
+    if (deg < kEll) {
+      // off-diagonal entries first, then the diagonal's mark (at most slot 14)
+      const int32_t v = (int32_t)(g - row_base[nmats == 1 ? 0 : find_segment(row_base, nmats, g)]);
+      int k = 0;
+      bool diag = false;
+      for (int p = 0; p < deg; ++p) {
+        const int32_t c = adj[b + p];
+        if (c == v) diag = true;
+        else slot[k++] = c;
+      }
+      if (diag) slot[k] = kDiag;
+    } else if (deg == kEll) {
+      for (int k = 0; k < kEll; ++k) slot[k] = adj[b + k];
     } else {
       for (int k = 0; k < kEll - 1; ++k) slot[k] = adj[b + k];
       slot[kEll - 1] = -(b + (kEll - 1) + 2);
@@ -312,6 +324,14 @@ __device__ __forceinline__ void load_row(const MatDev& M, int dir, int32_t v, Ro
   }
 }
 
+// 1 when the row's slots mark its diagonal (kDiag): one more entry for degree counts.
+__device__ __forceinline__ uint32_t diag_mark(const RowCursor& rc) {
+  bool d = false;
+#pragma unroll
+  for (int k = 0; k < kEll - 1; ++k) d = d || rc.e[k] == kDiag;
+  return d ? 1u : 0u;
+}
+
 // Window bits of direction 1: column j = src, row m > j, same supernode chunk, m - j - 1 < 128.
 // Selects instead of win[t >> 5]: a dynamically indexed register array would live in scratch.
 __device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t m, int32_t chunk_end) {
@@ -428,7 +448,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       bool has_prev_in_row = false;
       if (src > s0) {
         has_prev_in_row = sm == src - 1;
-        if (sm <= -2)
+        if (lane == kEll - 1 && sm <= -2)  // slot 15: a CSR tail
           for (int32_t p = -(sm + 2), pe = gload(M.rowptr[dir] + src + 1); p < pe; ++p)
             has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
@@ -474,7 +494,8 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
         // (The wide path below would run all its slot groups for one or two active lanes.)
         auto claim_one = [&](int32_t m, bool seedpass) {
           const bool valid = m >= 0;
-          const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+          // entries of the row, with the diagonal's mark
+          const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid || m == kDiag));
           const uint32_t nup = (uint32_t)__popcll(__ballot(valid && m > src));
           if (seedpass) {
             deg0 += nvalid;
@@ -549,7 +570,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             load_row(M, dir, v, rc);
             vesc_any = v >= 0 && gload(M.esc[dir] + v) != 0;  // tested after the claims
           }
-          uint32_t deg = 0;
+          uint32_t deg = diag_mark(rc);
           // kLightStep slots at a time: their home-slot CAS operations are in flight together;
           // a collision probes on. New entries below src are queued with one wave scan.
           auto claim = [&](int32_t m, int32_t old, uint32_t home) -> int {
@@ -1269,7 +1290,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     ov_end = gload(M.rowptr[d] + v + 1);
                     m = kPad;
                   }
-                  const uint32_t valid = m >= 0 ? 1u : 0u;
+                  const uint32_t valid = (m >= 0 || m == kDiag) ? 1u : 0u;  // an entry, or the diagonal's mark
+                  if (m == kDiag) m = kPad;
                   if (seed) {
                     deg0 += valid + dupv;
                     u_seed += (m > src) ? 1u : 0u;  // entries of A's row above the diagonal
@@ -1321,7 +1343,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
                     rc.ov_pos = rc.ov_end = 0;
                   }
-                  uint32_t deg = 0;
+                  uint32_t deg = diag_mark(rc);
                   uint32_t mask_r = 0, mask_u = 0;
 #pragma unroll
                   for (int h = 0; h < kEll; h += 8) {
@@ -1696,7 +1718,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   for (int k = 0; k < kEll; ++k) rc.e[k] = kPad;
                   rc.ov_pos = rc.ov_end = 0;
                 }
-                uint32_t deg = 0;
+                uint32_t deg = diag_mark(rc);
                 // Claim the bits of 8 slots at a time: the LDS atomics are independent and in
                 // flight together. Rows are packed, so the second group is skipped when no lane
                 // has a ninth entry. New entries below src go to the next level, new entries above
@@ -1908,8 +1930,10 @@ hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_ell_build(int64_t nrows, const int32_t* rowptr, const int32_t* adj, int32_t* ell, hipStream_t s) {
-  hipLaunchKernelGGL(k_ell_build, dim3(grid_for(nrows, 256, 16384)), dim3(256), 0, s, nrows, rowptr, adj, ell);
+hipError_t launch_ell_build(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
+                            const int32_t* adj, int32_t* ell, hipStream_t s) {
+  hipLaunchKernelGGL(k_ell_build, dim3(grid_for(nrows, 256, 16384)), dim3(256), 0, s, nrows, nmats, row_base, rowptr,
+                     adj, ell);
   return hipGetLastError();
 }
 
