@@ -332,6 +332,32 @@ __device__ __forceinline__ uint32_t diag_mark(const RowCursor& rc) {
   return d ? 1u : 0u;
 }
 
+// Whether some entry u < src - 1 of row src (its ELL slots, one per lane 0..15 in sm) holds
+// src - 1 in its own ELL slots: then src -> u -> src - 1 stays below src, so src - 1 is in R_src
+// and X_src contains X_{src-1}. A sufficient test (CSR tails are not searched); 16 lanes per
+// candidate row, four candidates per load round. Wave-uniform call.
+__device__ __forceinline__ bool two_hop_prev(const int32_t* __restrict__ ell, int32_t src, int32_t sm) {
+  const int lane = lane_id();
+  uint64_t cand = __ballot(lane < kEll && sm >= 0 && sm < src - 1);
+  while (cand != 0) {
+    int32_t u[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      u[c] = -1;
+      if (cand != 0) {
+        const int p = __ffsll((unsigned long long)cand) - 1;
+        cand &= cand - 1;
+        u[c] = __builtin_amdgcn_readlane(sm, p);
+      }
+    }
+    const int ci = lane >> 4;
+    const int32_t mine = ci == 0 ? u[0] : ci == 1 ? u[1] : ci == 2 ? u[2] : u[3];
+    const int32_t e = mine >= 0 ? gload(ell + (int64_t)mine * kEll + (lane & (kEll - 1))) : kPad;
+    if (__ballot(e == src - 1) != 0) return true;
+  }
+  return false;
+}
+
 // Window bits of direction 1: column j = src, row m > j, same supernode chunk, m - j - 1 < 128.
 // Selects instead of win[t >> 5]: a dynamically indexed register array would live in scratch.
 __device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t m, int32_t chunk_end) {
@@ -452,7 +478,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           for (int32_t p = -(sm + 2), pe = gload(M.rowptr[dir] + src + 1); p < pe; ++p)
             has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
-      const bool chained = __ballot(has_prev_in_row) != 0;
+      bool chained = __ballot(has_prev_in_row) != 0;
+      // otherwise, through an entry whose own row holds src-1 (when X_{src-1} is at hand or src-1
+      // escalated)
+      if (!chained && src > s0 && (carry_ok || prev_esc)) chained = two_hop_prev(M.ell[dir], src, sm);
       // an entry u < src (ELL slots) whose row escalated already: X_src contains X_u. A flag
       // written by another wave may not be seen yet; that only costs the attempt. Read here,
       // tested after the source row's pass (which hides the load).
@@ -1028,6 +1057,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         // the window's escalated rows as bits (rows w0 .. we - 1, at most 64), for the leader
         const uint64_t em = __ballot(tid < we - w0 && esc_v != 0);
         if (tid == 0) s_escm[dd] = em;
+      }
+      bsync();
+      // Rows that do not hold their predecessor directly but through an entry u < row - 1 whose
+      // own row holds row - 1 (row -> u -> row - 1 stays below row): they chain too. Tested for
+      // escalated rows whose predecessor is escalated, i.e. traversed here just before them.
+      for (int di = 0; di < ndir; ++di) {
+        const int dd = paired ? di : item_dir;
+        const int32_t* const stash = (kPairedAny && dd) ? stash1 : stash0;
+#pragma unroll 1
+        for (int q = 0; q < kPfRounds; ++q) {
+          const int32_t i = q * kThreads + tid;
+          const int32_t li2 = i >> 4;
+          const int32_t row = w0 + li2;
+          bool hit = false;
+          if (i < np * kEll && li2 >= 1 && s_esc[dd][li2] != 0 && s_esc[dd][li2 - 1] != 0 && s_hold[dd][li2] == 0) {
+            const int32_t u = stash[i];
+            if (u >= 0 && u < row - 1) {
+              const int32_t* const ur = M.ell[dd] + (int64_t)u * kEll;
+#pragma unroll
+              for (int q4 = 0; q4 < 4; ++q4) {
+                const v4i32 t4 = gload_v4(ur + 4 * q4);
+                hit = hit || t4.x == row - 1 || t4.y == row - 1 || t4.z == row - 1 || t4.w == row - 1;
+              }
+            }
+          }
+          const uint64_t b = __ballot(hit);
+          if ((i & (kEll - 1)) == 0 && i < np * kEll && ((b >> (lane & 48)) & 0xFFFFull) != 0) s_hold[dd][li2] = 1;
+        }
       }
       bsync();
 
