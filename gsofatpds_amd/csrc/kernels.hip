@@ -396,6 +396,17 @@ __device__ __forceinline__ int hash_probe(int32_t* H, int32_t m, uint32_t h) {
   }
 }
 __device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_probe(H, m, hash_home(m)); }
+// Membership without inserting (the table always keeps an empty slot).
+__device__ __forceinline__ bool hash_has(const int32_t* H, int32_t m) {
+  uint32_t h = hash_home(m);
+  for (int k = 0; k < kHash; ++k) {
+    const int32_t x = H[h];
+    if (x == m) return true;
+    if (x == -1) return false;
+    h = (h + 1) & (kHash - 1);
+  }
+  return false;
+}
 
 // One wave per item = a block of M.light_block consecutive rows of one direction, in increasing
 // order. Row s reuses the previous row's hash set when row s holds s-1 (then X_{s-1} is part of
@@ -411,6 +422,15 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
   int32_t* H = s_hash[wave];
   int32_t* Q = s_q[wave];
   for (int i = lane; i < kHash; i += 64) H[i] = -1;
+  // Sibling exchange. Waves 2i and 2i+1 run the two directions of one block (items 2j and
+  // 2j+1). After each row t a wave publishes whether t+1 is in its X_t. For the other direction
+  // that decides exactly whether t is in R_{t+1}: a path t+1 -> ... -> t through vertices below
+  // t in one direction is the reverse of a path t -> ... -> t+1 in the other, i.e. t+1 in U_t.
+  __shared__ unsigned long long s_pub[kLightWaves];  // (item << 32) | (row + 1) of the last row published
+  __shared__ uint8_t s_nxt[kLightWaves][4];          // by row mod 4: 1 in, 2 not in, 3 unknown
+  if (threadIdx.x < kLightWaves) s_pub[threadIdx.x] = 0ull;
+  __syncthreads();
+  const int sib = wave ^ 1;
 
   // per-wave running totals, flushed once per matrix (one atomic per wave and slot)
   uint64_t t_fill = 0, t_w = 0, t_r = 0, t_nnz_u = 0, t_nnz_l = 0;
@@ -479,9 +499,29 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             has_prev_in_row |= gload(M.adj[dir] + p) == src - 1;
       }
       bool chained = __ballot(has_prev_in_row) != 0;
-      // otherwise, through an entry whose own row holds src-1 (when X_{src-1} is at hand or src-1
-      // escalated)
-      if (!chained && src > s0 && (carry_ok || prev_esc)) chained = two_hop_prev(M.ell[dir], src, sm);
+      // otherwise (when X_{src-1} is at hand or src-1 escalated): from the sibling wave's row
+      // src-1 of the other direction, or through an entry whose own row holds src-1
+      if (!chained && src > s0 && (carry_ok || prev_esc)) {
+        int st = 3;
+        const unsigned long long want = ((unsigned long long)(k ^ 1) << 32) | (uint32_t)src;
+        for (int spin = 0; spin < (1 << 16); ++spin) {  // bounded: an unknown answer is also correct
+          const unsigned long long p = __hip_atomic_load(&s_pub[sib], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (p >= want) {
+            // published row src-1 of this block (or went past it): read it while it is in the ring
+            if ((p >> 32) == (unsigned long long)(k ^ 1) && (uint32_t)p - (uint32_t)src < 4u) {
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+              st = s_nxt[sib][(src - 1) & 3];
+              const unsigned long long p2 = __hip_atomic_load(&s_pub[sib], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if ((p2 >> 32) != (unsigned long long)(k ^ 1) || (uint32_t)p2 - (uint32_t)src >= 4u) st = 3;
+            }
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        st = __builtin_amdgcn_readfirstlane(st);
+        if (st == 1) chained = true;
+        else if (st != 2) chained = two_hop_prev(M.ell[dir], src, sm);
+      }
       // an entry u < src (ELL slots) whose row escalated already: X_src contains X_u. A flag
       // written by another wave may not be seen yet; that only costs the attempt. Read here,
       // tested after the source row's pass (which hides the load).
@@ -745,6 +785,16 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
         carry_ok = false;
         prev_esc = true;
       }
+      // publish row src for the sibling wave: is src+1 in X_src?
+      {
+        const uint8_t st = esc ? 3 : (src + 1 < s1 && hash_has(H, src + 1)) ? 1 : 2;
+        if (lane == 0) {
+          s_nxt[wave][src & 3] = st;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __hip_atomic_store(&s_pub[wave], ((unsigned long long)k << 32) | (uint32_t)(src + 1), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
     }
     if (hcount > 0) {
       for (int i = lane; i < kHash; i += 64) H[i] = -1;
@@ -752,6 +802,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     }
   }
   flush(last_mat);
+  if (lane == 0) __hip_atomic_store(&s_pub[wave], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // done
 }
 
 // ------------------------------------------------------------------------------------------
