@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
   // that decides exactly whether t is in R_{t+1}: a path t+1 -> ... -> t through vertices below
   // t in one direction is the reverse of a path t -> ... -> t+1 in the other, i.e. t+1 in U_t.
   __shared__ unsigned long long s_pub[kLightWaves];  // (item << 32) | (row + 1) of the last row published
-  __shared__ uint8_t s_nxt[kLightWaves][4];          // by row mod 4: 1 in, 2 not in, 3 unknown
+  __shared__ uint8_t s_nxt[kLightWaves][8];          // by row mod 8: 1 in, 2 not in, 3 unknown
   if (threadIdx.x < kLightWaves) s_pub[threadIdx.x] = 0ull;
   __syncthreads();
   const int sib = wave ^ 1;
@@ -507,12 +507,15 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
         for (int spin = 0; spin < (1 << 16); ++spin) {  // bounded: an unknown answer is also correct
           const unsigned long long p = __hip_atomic_load(&s_pub[sib], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (p >= want) {
-            // published row src-1 of this block (or went past it): read it while it is in the ring
-            if ((p >> 32) == (unsigned long long)(k ^ 1) && (uint32_t)p - (uint32_t)src < 4u) {
+            // Published row src-1 of this block (or went past it). Its ring slot is rewritten when
+            // the sibling publishes row src+7, which it may be doing once row src+6 is out: read
+            // the slot only while the sibling has published at most row src+5, before and after.
+            if ((p >> 32) == (unsigned long long)(k ^ 1) && (uint32_t)p - (uint32_t)src < 7u) {
               __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-              st = s_nxt[sib][(src - 1) & 3];
+              st = s_nxt[sib][(src - 1) & 7];
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
               const unsigned long long p2 = __hip_atomic_load(&s_pub[sib], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if ((p2 >> 32) != (unsigned long long)(k ^ 1) || (uint32_t)p2 - (uint32_t)src >= 4u) st = 3;
+              if ((p2 >> 32) != (unsigned long long)(k ^ 1) || (uint32_t)p2 - (uint32_t)src >= 7u) st = 3;
             }
             break;
           }
@@ -789,7 +792,9 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       {
         const uint8_t st = esc ? 3 : (src + 1 < s1 && hash_has(H, src + 1)) ? 1 : 2;
         if (lane == 0) {
-          s_nxt[wave][src & 3] = st;
+          // the previous row's publication is visible before its ring slot's next use
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          s_nxt[wave][src & 7] = st;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           __hip_atomic_store(&s_pub[wave], ((unsigned long long)k << 32) | (uint32_t)(src + 1), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
