@@ -449,6 +449,25 @@ def test_full_size_structure_equals_oracle(name, wl):
     assert oracle.structure_digests(up, ui, lp, li) == (gd["U_rows_sha256"], gd["L_cols_sha256"])
 
 
+@pytest.mark.parametrize("name,wl", [("grid3d_110592", g.WL_GRID3D_110592), ("grid2d_659033", g.WL_GRID2D_659033)])
+def test_repeated_runs_equal_oracle(name, wl):
+    """Which rows chain, escalate or take the workgroup path depends on timing (escalation flags
+    and the wave path's sibling exchange are read while other waves write them); results must
+    not. Several runs of one plan, each against the oracle's committed totals and digests."""
+    gd, hashlib = _large(name)
+    o = gd["oracle_S128"]
+    p = g.generate_workload(wl)
+    with g.Plan(p, chunk_size=128, structure=0) as plan:
+        for k in range(6):
+            say(f"[repeated] {name} run {k}")
+            plan.run()
+            r = plan.result()
+            assert (r.fill_count, r.nnz_F, r.edge_checks, r.reach, r.supernodes_counted) == \
+                (o["fill"], o["nnz_F"], o["W"], o["R"], o["nsup"])
+            assert hashlib.sha256(plan.nzu().astype(np.int32).tobytes()).hexdigest() == gd["nzu_sha256_S128"]
+            assert hashlib.sha256(plan.supernode_flags().tobytes()).hexdigest() == gd["sup_flag_sha256_S128"]
+
+
 def test_full_size_structure_properties():
     """Size-independent properties of the explicit structure at 659,033 rows: counts agree with
     the per-row counts, rows are sorted and duplicate-free, every entry of A is in F, and runs
