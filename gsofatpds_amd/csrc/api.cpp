@@ -30,6 +30,7 @@ Options to_options(const gsofa_options* o) {
   r.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   r.autotune = o->autotune ? 1 : 0;
   r.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
+  r.split_blocks = o->split_blocks < 0 ? -1 : o->split_blocks;
   return r;
 }
 

@@ -45,6 +45,68 @@ int prepare(const gsofa_csr* a, Prepared& out) {
   return GSOFA_OK;
 }
 
+// Elimination tree of A + A^T by Liu's algorithm with path compression: every pair {k, i}, k < i,
+// of A's off-diagonal entries is taken at step i. Then subtree sizes.
+static void subtree_sizes(const Prepared& P, std::vector<int32_t>& size) {
+  const int32_t n = P.n;
+  std::vector<int32_t> cnt((size_t)n + 1, 0);
+  for (int32_t r = 0; r < n; ++r)
+    for (int32_t k = P.rowptr[r]; k < P.rowptr[r + 1]; ++k) {
+      const int32_t c = P.adj[k];
+      if (c != r) ++cnt[(size_t)std::max(c, r) + 1];
+    }
+  for (int32_t i = 0; i < n; ++i) cnt[(size_t)i + 1] += cnt[i];
+  std::vector<int32_t> lower((size_t)cnt[n]);
+  {
+    std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
+    for (int32_t r = 0; r < n; ++r)
+      for (int32_t k = P.rowptr[r]; k < P.rowptr[r + 1]; ++k) {
+        const int32_t c = P.adj[k];
+        if (c != r) lower[(size_t)cur[std::max(c, r)]++] = std::min(c, r);
+      }
+  }
+  std::vector<int32_t> parent((size_t)n, -1), anc((size_t)n, -1);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t p = cnt[i]; p < cnt[(size_t)i + 1]; ++p) {
+      int32_t r = lower[p];
+      while (anc[r] != -1 && anc[r] != i) {
+        const int32_t next = anc[r];
+        anc[r] = i;
+        r = next;
+      }
+      if (anc[r] == -1) {
+        anc[r] = i;
+        parent[r] = i;
+      }
+    }
+  size.assign((size_t)n, 1);
+  for (int32_t v = 0; v < n; ++v)
+    if (parent[v] >= 0) size[parent[v]] += size[v];
+}
+
+int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std::vector<int32_t>& cut) {
+  const int32_t n = P.n;
+  const int64_t nb = ((int64_t)n + chain_block - 1) / chain_block;
+  cut.assign((size_t)nb, -1);
+  const int64_t min_size = std::max<int64_t>(16 * (int64_t)chain_block, n / std::max(divisor, 1));
+  if (chain_block < 2 || n < 2 * min_size) return 0;
+  if (P.subtree.empty()) subtree_sizes(P, P.subtree);
+  const std::vector<int32_t>& size = P.subtree;
+  int64_t count = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int32_t r0 = (int32_t)(b * chain_block), r1 = (int32_t)std::min<int64_t>(n, r0 + (int64_t)chain_block);
+    int32_t best = -1;
+    for (int32_t s = r0 + 1; s < r1; ++s)
+      if (2 * (int64_t)size[s] >= 3 * (int64_t)size[s - 1] && size[s] >= min_size && (best < 0 || size[s] > size[best]))
+        best = s;
+    if (best >= 0) {
+      cut[(size_t)b] = best;
+      ++count;
+    }
+  }
+  return count;
+}
+
 // Paired items pay a bitmap save per row and win where chaining by "row s holds s-1" is rare
 // (unstructured patterns); structured ones (stencils in nested-dissection order) chain directly.
 bool pair_directions(const Prepared& P, int32_t mode) {
@@ -300,6 +362,21 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     blkq_base_h_[b + 1] = blkq_base_h_[b] + ((int64_t)ns_[b] + opt_.chain_block - 1) / opt_.chain_block;
   nblk_total_ = blkq_base_h_[nmats_];
   nb_max_ = (int32_t)(((int64_t)n_max_ + opt_.chain_block - 1) / opt_.chain_block);
+  // blocks cut at large separator starts (their parts are separate workgroup-path items)
+  std::vector<int32_t> cut_all;
+  std::vector<bool> has_cut((size_t)nmats_, false);
+  int64_t ncut = 0;
+  if (opt.split_blocks != -1) {
+    std::vector<int32_t> cut;
+    for (int b = 0; b < nmats_; ++b) {
+      const int64_t c = block_cuts(*mats[b], opt_.chain_block, opt.split_blocks > 0 ? opt.split_blocks : kCutDivisor, cut);
+      if (c == 0) continue;
+      if (cut_all.empty()) cut_all.assign((size_t)nblk_total_, -1);
+      std::copy(cut.begin(), cut.end(), cut_all.begin() + blkq_base_h_[b]);
+      has_cut[b] = true;
+      ncut += c;
+    }
+  }
   for (int b = 0; b < nmats_; ++b)
     item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)ns_[b] + light_block_ - 1) / light_block_);
   nitems_ = item_base_h_[nmats_];
@@ -325,7 +402,11 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(row_base_d_, nmats_ + 1, "row base"));
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
   GSOFA_TRY(alloc(ws_, 1, "work state"));
-  GSOFA_TRY(alloc(heavy_items_, 2 * nblk_total_, "escalation list"));
+  GSOFA_TRY(alloc(heavy_items_, 2 * (nblk_total_ + ncut), "escalation list"));
+  if (ncut > 0) {
+    GSOFA_TRY(alloc(cut_, (size_t)nblk_total_, "block cuts"));
+    GSOFA_TRY(upload(cut_, cut_all.data(), cut_all.size() * 4, "upload block cuts"));
+  }
   // per resident workgroup: two frontier spill queues, the U list (beyond its LDS part) and the
   // saved bitmap of the last row (with paired items: a U list per direction and a ring of the
   // last saved rows' bitmaps per direction)
@@ -401,6 +482,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.esc[1] = esc_ + nrows_ + rb;
     M.blk_flag[0] = blk_flag_ + blkq_base_h_[b];
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
+    M.split = has_cut[b] ? cut_ + blkq_base_h_[b] : nullptr;
     M.chain_block = opt_.chain_block;
     M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
     M.light_block = light_block_;
@@ -435,6 +517,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * nrows * kEll * 2;          // ELL A, A^T
   b += 4 * nrows * 3 + 16 * nrows;    // column counts, U/L counts, window bits
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
+  b += 4 * nblk + 8 * 2 * nblk;                           // block cuts and their extra items
   b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
   b += 4 * heavy_grid * heavy_region_ints(nmax, paired);  // spill
   if (place_bitmaps_in_hbm(nmax, opt, info)) b += 4 * (int64_t)cus * (((int64_t)nmax + 31) / 32);  // one per CU

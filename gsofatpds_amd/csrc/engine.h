@@ -22,7 +22,17 @@ struct Prepared {
   std::vector<int32_t> dup_u;   // empty when the input had no duplicate upper entries
   std::vector<int32_t> dup_all; // input row length minus de-duplicated length; empty if none
   int64_t hold_prev = 0;        // rows s >= 1 whose row holds s-1
+  // Subtree sizes of the elimination tree of A + A^T (|T[s]| bounds |R_s| + 1), computed on
+  // first use (block cuts) and kept.
+  mutable std::vector<int32_t> subtree;
 };
+
+// Rows where chain blocks are cut into two workgroup-path items: in each block the row s whose
+// elimination subtree is at least 1.5 times its predecessor's and at least
+// max(16 * chain_block, n / divisor) rows (the start of a separator over a large subtree: a
+// traversal of that subtree begins there), the largest such; -1 where none. Returns the count.
+int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std::vector<int32_t>& cut);
+constexpr int32_t kCutDivisor = 64;  // the automatic divisor
 
 // Whether the workgroup path runs both directions of a block together (Options::pair_directions).
 bool pair_directions(const Prepared& P, int32_t mode);
@@ -44,6 +54,8 @@ struct Options {
   int32_t bitmap_in_hbm = 0;    // bitmap-path bitmaps in HBM: 1 always, 0 only when LDS is too small
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
+  int32_t split_blocks = 0;     // cut blocks at large separator starts (block_cuts): 0 automatic
+                                // (kCutDivisor), k > 0 subtrees of at least n / k rows, -1 never
 };
 
 struct MatResult {
@@ -162,6 +174,7 @@ class DeviceSet {
   int32_t* spill_ = nullptr;
   uint8_t* esc_ = nullptr;        // 2 * nrows escalation flags (padded to a multiple of 4)
   uint32_t* blk_flag_ = nullptr;  // 2 * nblk_total
+  int32_t* cut_ = nullptr;        // nblk_total: block cut rows (block_cuts), when any
   LaunchConfig cfg_;
   int64_t nblk_total_ = 0, esc_words_ = 0;
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks

@@ -58,6 +58,8 @@ struct MatDev {
   unsigned long long* acc;        // kAccSlots
   uint8_t* esc[2];                // n: 1 where the wave path escalated the source
   uint32_t* blk_flag[2];          // ceil(n / chain_block): 1 where the block has an escalated row
+  const int32_t* split;           // ceil(n / chain_block): a row where the block is cut into two
+                                  // workgroup-path items, or -1 (null: no block is cut)
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
   int32_t paired;                 // 1: workgroup-path items run both directions of a block
   int32_t light_block;            // rows per wave-path item
@@ -72,9 +74,10 @@ struct WorkState {
   unsigned long long pad;
 };
 
-// Item encoding for the escalation list: matrix index, direction, block of chain_block rows.
-__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t block) {
-  return ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)block;
+// Item encoding for the escalation list: part of a cut block (bit 63), matrix index (bits
+// 33..62), direction, block of chain_block rows.
+__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t block, uint32_t part = 0) {
+  return ((uint64_t)part << 63) | ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)block;
 }
 
 }  // namespace gsofa

@@ -838,8 +838,9 @@ __global__ void k_heavy_hist(const MatDev* __restrict__ mats, int64_t nblk_total
     uint32_t dir;
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
-    atomicAdd(&bucket[nb_max - 1 - b], 1);
-    atomicAdd(&ws->heavy_count, 1ull);
+    const int parts = (mats[m].split && mats[m].split[b] >= 0) ? 2 : 1;
+    atomicAdd(&bucket[nb_max - 1 - b], parts);
+    atomicAdd(&ws->heavy_count, (unsigned long long)parts);
   }
 }
 
@@ -852,8 +853,15 @@ __global__ void k_heavy_scatter(const MatDev* __restrict__ mats, int64_t nblk_to
     uint32_t dir;
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
-    const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
-    heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b);
+    if (mats[m].split && mats[m].split[b] >= 0) {
+      // a cut block: the part from the cut on (the larger subtree) goes first
+      const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 2);
+      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b, 1u);
+      heavy_items[pos + 1] = encode_item((uint32_t)m, dir, (uint32_t)b, 0u);
+    } else {
+      const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
+      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b);
+    }
   }
 }
 
@@ -1045,16 +1053,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const unsigned long long idx = s_idx;
     if (idx >= total) break;
     const uint64_t item = heavy_items[idx];
-    const int mb = (int)(item >> 33);
+    const int mb = (int)((item >> 33) & 0x3fffffffu);
     const int item_dir = (int)((item >> 32) & 1);
     const int32_t blk = (int32_t)(item & 0xffffffffu);
+    const int part = (int)(item >> 63);
     const MatDev& M = mats[mb];
     const bool paired = kPairedAny && M.paired != 0;
     const int ndir = paired ? 2 : 1;
     const int32_t n = M.n;
     const int32_t bw = (n + 31) >> 5;
-    const int32_t r0 = blk * M.chain_block;
-    const int32_t r1 = min(n, r0 + M.chain_block);
+    int32_t r0 = blk * M.chain_block;
+    int32_t r1 = min(n, r0 + M.chain_block);
+    if (M.split) {  // a cut block: part 0 runs the rows before the cut, part 1 the rest
+      const int32_t cut = M.split[blk];
+      if (cut > r0 && cut < r1) {
+        if (part) r0 = cut;
+        else r1 = cut;
+      }
+    }
     const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
     // The leader's state of this item, per direction (index = direction; a plain item uses its

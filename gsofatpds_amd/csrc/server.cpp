@@ -65,7 +65,8 @@ bool same_options(const Options& a, const Options& b) {
   return a.chunk == b.chunk && a.n_processes == b.n_processes && a.structure == b.structure &&
          a.device == b.device && a.heavy_threads == b.heavy_threads && a.heavy_wgs_per_cu == b.heavy_wgs_per_cu &&
          a.light_wgs_per_cu == b.light_wgs_per_cu && a.qcap == b.qcap && a.chain_block == b.chain_block &&
-         a.pair_directions == b.pair_directions && a.bitmap_in_hbm == b.bitmap_in_hbm && a.cu_share == b.cu_share;
+         a.pair_directions == b.pair_directions && a.bitmap_in_hbm == b.bitmap_in_hbm && a.cu_share == b.cu_share &&
+         a.split_blocks == b.split_blocks;
 }
 
 // Host bytes a request keeps alive (its prepared pattern).
@@ -342,6 +343,7 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 0;
   r->opt.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   r->opt.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
+  r->opt.split_blocks = -1;  // the cuts need the elimination tree: not on the admission path
   r->opt.structure = 0;
   DeviceInfo info;
   int rc = device_info(srv->device, info);

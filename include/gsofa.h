@@ -151,6 +151,11 @@ typedef struct gsofa_options {
                                0 (default): only where one does not fit in LDS (n above about
                                1.2 M rows), so any n runs. The reference's labels live in global
                                memory at every n (SOA:470-479).                                */
+  int32_t split_blocks;     /* a workgroup-path block that holds the start of a separator over
+                               a large subtree (its elimination subtree at least 1.5x its
+                               predecessor's and n / k rows) runs as two items, so two
+                               traversals of whole subtrees do not queue in one workgroup:
+                               0 (default) k = 64, k > 0 that k, -1 never                      */
 } gsofa_options;
 
 typedef struct gsofa_result {

@@ -468,6 +468,35 @@ def test_repeated_runs_equal_oracle(name, wl):
             assert hashlib.sha256(plan.supernode_flags().tobytes()).hexdigest() == gd["sup_flag_sha256_S128"]
 
 
+@pytest.mark.parametrize("case", ["grid2d_nd_256", "grid3d_14_chain3", "ds_11948", "grid3d_110592"])
+def test_block_cuts_change_nothing(case):
+    """Blocks cut at large separator starts run as two workgroup-path items; results are those of
+    uncut blocks (and of the oracle where it is cheap), with the structure."""
+    if case == "grid2d_nd_256":
+        p, opt, with_oracle = g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=256, ny=256, order=g.ORDER_ND,
+                                         drop_side=g.DROP_LOWER, drop_ppm=100000, seed=5), {}, True
+    elif case == "grid3d_14_chain3":
+        p, opt, with_oracle = g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=14, ny=14, nz=14, order=g.ORDER_ND,
+                                         seed=8), {"chain_block": 3}, True
+    elif case == "ds_11948":
+        p, opt, with_oracle = load_ds(), {}, True
+    else:
+        p, opt, with_oracle = g.generate_workload(g.WL_GRID3D_110592), {}, False
+    say(f"[cuts] {case} n={p.n}")
+    if with_oracle:
+        check(p, label=f"{case} cut", **opt)
+    out = []
+    for split in (0, -1):
+        with g.Plan(p, structure=1, split_blocks=split, **opt) as plan:
+            plan.run()
+            r = plan.result()
+            up, ui, lp, li = plan.structure()
+            out.append(((r.fill_count, r.nnz_F, r.edge_checks, r.reach, r.supernodes_counted),
+                        oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
+                        g.fnv1a_array(plan.supernode_flags())))
+    assert out[0] == out[1]
+
+
 def test_full_size_structure_properties():
     """Size-independent properties of the explicit structure at 659,033 rows: counts agree with
     the per-row counts, rows are sorted and duplicate-free, every entry of A is in F, and runs
