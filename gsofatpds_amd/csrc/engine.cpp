@@ -263,7 +263,8 @@ int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Option
     int64_t k = (int64_t)n_max / std::max<int64_t>(1, 2 * (int64_t)out.heavy_grid);
     out.chain_block = (int32_t)std::max<int64_t>(4, std::min<int64_t>(cap, k));
   }
-  const int occ_l = light_occupancy();
+  out.light_bits = n_max > kLightBigRows ? kLightBitsBig : kLightBits;
+  const int occ_l = light_occupancy(out.light_bits);
   if (occ_l <= 0) return fail(GSOFA_ERR_HIP, "k_light occupancy query failed");
   out.light_grid = cus * (opt.light_wgs_per_cu ? std::min(opt.light_wgs_per_cu, occ_l) : occ_l);
   const int64_t waves = (int64_t)out.light_grid * kLightWaves;
@@ -293,7 +294,7 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
             "transpose");
   GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
-  GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, s), "k_light");
+  GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built
   GSOFA_SEQ(launch_heavy_order(q.mats, q.nblk_total, q.nmats, q.blkq_base, q.blk_flag, q.nb_max, q.bucket, q.cursor,
                                q.block_sums, q.total, q.ws, q.heavy_items, s),
@@ -486,7 +487,8 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.chain_block = opt_.chain_block;
     M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
     M.light_block = light_block_;
-    M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall : kLightLimitMax;
+    M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall
+                    : cfg_.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");

@@ -77,6 +77,7 @@ int device_info(int device, DeviceInfo& info);
 // block sizes of both paths.
 struct LaunchConfig {
   int heavy_threads = 1024, heavy_grid = 0, heavy_wgs_per_cu = 0, light_grid = 0;
+  int light_bits = kLightBits;  // wave path hash set size (kLightBits or kLightBitsBig)
   int32_t qcap = 0, ucap = 0, bw_max = 0, chain_block = 0, light_block = kLightBlockMin;
   bool hbm_bitmap = false;
   size_t heavy_lds = 0;  // dynamic LDS bytes of k_heavy

@@ -96,7 +96,8 @@ hipError_t launch_ell_build(int64_t nrows, int nmats, const int64_t* row_base, c
 hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
                             const int32_t* adj, int32_t* col_cnt, int32_t* rowptr_t, int32_t* cursor,
                             int32_t* adj_t, int32_t* block_sums, int32_t* total, hipStream_t s);
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid,
+// bits: kLightBits, or kLightBitsBig (a hash set of 2,048 slots and an escalation limit of 1,024)
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid, int bits,
                         hipStream_t s);
 // The escalated blocks (blk_flag, 2 * nblk_total slots) into heavy_items in descending block
 // order; sets ws->heavy_count. bucket has nb_max entries, cursor nb_max + 1, block_sums
@@ -144,11 +145,17 @@ constexpr int kLightBlockMin = 8, kLightBlockMax = 64;
 // with 256; 110,592 rows 2.89 ms with 512, 3.02 with 256; 659,033 rows 8.4 / 9.4 ms.
 constexpr int kLightLimitMax = 512;
 constexpr int kLightLimitSmall = 256, kLightSmallRows = 32768;
+// Wave path hash sets: 1,024 slots (five workgroups per CU), or, for sets whose largest matrix
+// has more than kLightBigRows rows, 2,048 slots with the limit at 1,024 reached vertices (three
+// workgroups per CU). Measured: the 659,033-row stencil 7.46 -> 6.78 ms per step with the
+// structure (rows of 512-1,024 reached vertices chain cheaply on the wave path instead of taking
+// a workgroup row each), the 110,592-row one 2.58 -> 2.75 ms counting only (occupancy).
+constexpr int kLightBits = 10, kLightBitsBig = 11, kLightLimitBig = 1024, kLightBigRows = 262144;
 }  // namespace gsofa
 
 namespace gsofa {
 // Resident workgroups of k_heavy per CU for a block size and dynamic LDS size.
 int heavy_occupancy(int threads, size_t lds_bytes, bool hbm_bitmap);
 hipError_t launch_copy16(const void* src, void* dst, int64_t n16, int grid, hipStream_t s);
-int light_occupancy();
+int light_occupancy(int bits);
 }  // namespace gsofa

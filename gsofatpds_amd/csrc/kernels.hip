@@ -375,19 +375,26 @@ __device__ __forceinline__ void window_bit(uint32_t win[4], int32_t src, int32_t
 // ------------------------------------------------------------------------------------------
 // k_light: one wave per source, LDS hash set of the reached vertices
 // ------------------------------------------------------------------------------------------
-constexpr int kHashBits = 10;
-constexpr int kHash = 1 << kHashBits;
-constexpr int kLightLimit = kLightLimitMax;  // LDS sizing: the largest escalation limit
 constexpr int kLightStep = 4;                // ELL slots claimed together between escalation checks
-constexpr int kLightQ = kLightLimit + 64 * kLightStep;  // R entries never exceed the limit plus one step
-static_assert(kLightLimit + 64 * kLightStep < kHash, "the hash set must never fill");
 constexpr int kLightVpp = 64 / kEll;         // vertices of a narrow pass (one ELL slot per lane)
 
-__device__ __forceinline__ uint32_t hash_home(int32_t m) { return ((uint32_t)m * 0x9E3779B1u) >> (32 - kHashBits); }
+// The hash set of a wave path variant: 2^kBits slots, escalation limit (LDS sizing) and queue.
+template <int kBits>
+struct LightHash {
+  static constexpr int kHash = 1 << kBits;
+  static constexpr int kLimit = kBits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
+  static constexpr int kQ = kLimit + 64 * kLightStep;  // R entries never exceed the limit plus one step
+  static_assert(kLimit + 64 * kLightStep < kHash, "the hash set must never fill");
+};
 
-// Linear probing from slot h. The table holds at most kLightLimit + 64 * kLightStep < kHash
+template <int kBits>
+__device__ __forceinline__ uint32_t hash_home(int32_t m) { return ((uint32_t)m * 0x9E3779B1u) >> (32 - kBits); }
+
+// Linear probing from slot h. The table holds at most kLimit + 64 * kLightStep < kHash
 // entries, so an empty slot is always found.
+template <int kBits>
 __device__ __forceinline__ int hash_probe(int32_t* H, int32_t m, uint32_t h) {
+  constexpr int kHash = 1 << kBits;
   while (true) {
     const int32_t old = atomicCAS(&H[h], -1, m);
     if (old == -1) return 1;
@@ -395,10 +402,13 @@ __device__ __forceinline__ int hash_probe(int32_t* H, int32_t m, uint32_t h) {
     h = (h + 1) & (kHash - 1);
   }
 }
-__device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_probe(H, m, hash_home(m)); }
+template <int kBits>
+__device__ __forceinline__ int hash_insert(int32_t* H, int32_t m) { return hash_probe<kBits>(H, m, hash_home<kBits>(m)); }
 // Membership without inserting (the table always keeps an empty slot).
+template <int kBits>
 __device__ __forceinline__ bool hash_has(const int32_t* H, int32_t m) {
-  uint32_t h = hash_home(m);
+  constexpr int kHash = 1 << kBits;
+  uint32_t h = hash_home<kBits>(m);
   for (int k = 0; k < kHash; ++k) {
     const int32_t x = H[h];
     if (x == m) return true;
@@ -413,8 +423,11 @@ __device__ __forceinline__ bool hash_has(const int32_t* H, int32_t m) {
 // X_s). A row that holds an escalated u < s escalates without an attempt: X_u, already over the
 // limit, is part of X_s.
 
-__global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(5))) void k_light(
+template <int kBits>
+__global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(kBits == kLightBitsBig ? 3 : 5))) void k_light(
     const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems) {
+  constexpr int kHash = LightHash<kBits>::kHash;
+  constexpr int kLightQ = LightHash<kBits>::kQ;
   __shared__ int32_t s_hash[kLightWaves][kHash];
   __shared__ int32_t s_q[kLightWaves][kLightQ];
   const int wave = threadIdx.x >> 6;
@@ -541,7 +554,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           uq = 0;
         }
         bool src_was_set = false;
-        if (lane == 0) src_was_set = hash_insert(H, src) == 0;
+        if (lane == 0) src_was_set = hash_insert<kBits>(H, src) == 0;
         src_was_set = __ballot(src_was_set) != 0;
         if (!src_was_set) ++hcount;
         if (track_u && carry && src_was_set) {
@@ -577,9 +590,9 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           }
           bool ins = false;
           if (valid) {
-            const uint32_t home = hash_home(m);
+            const uint32_t home = hash_home<kBits>(m);
             const int32_t old = atomicCAS(&H[home], -1, m);
-            ins = old == -1 || (old != m && hash_probe(H, m, (home + 1) & (kHash - 1)) != 0);
+            ins = old == -1 || (old != m && hash_probe<kBits>(H, m, (home + 1) & (kHash - 1)) != 0);
           }
           const bool is_r = ins && m < src;
           const uint64_t mr = __ballot(is_r);
@@ -648,7 +661,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           auto claim = [&](int32_t m, int32_t old, uint32_t home) -> int {
             if (old == -1) return 1;
             if (old == m) return 0;
-            return hash_probe(H, m, (home + 1) & (kHash - 1));
+            return hash_probe<kBits>(H, m, (home + 1) & (kHash - 1));
           };
 #pragma unroll
           for (int h = 0; h < kEll; h += kLightStep) {
@@ -658,7 +671,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
 #pragma unroll
             for (int j = 0; j < kLightStep; ++j) {
               const int32_t m = rc.e[h + j];
-              home[j] = hash_home(m);
+              home[j] = hash_home<kBits>(m);
               old[j] = m >= 0 ? atomicCAS(&H[home[j]], -1, m) : m;
             }
             uint32_t mr = 0, mu = 0, nnew = 0;
@@ -706,7 +719,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
             const bool active = m >= 0;
             deg += active;
             if (seed && m > src) ++u_seed;
-            const int ins = active ? hash_insert(H, m) : 0;
+            const int ins = active ? hash_insert<kBits>(H, m) : 0;
             const bool is_r = ins == 1 && m < src;
             const bool is_u = ins == 1 && m > src;
             const uint64_t mr = __ballot(is_r);
@@ -790,7 +803,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       }
       // publish row src for the sibling wave: is src+1 in X_src?
       {
-        const uint8_t st = esc ? 3 : (src + 1 < s1 && hash_has(H, src + 1)) ? 1 : 2;
+        const uint8_t st = esc ? 3 : (src + 1 < s1 && hash_has<kBits>(H, src + 1)) ? 1 : 2;
         if (lane == 0) {
           // the previous row's publication is visible before its ring slot's next use
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -2082,9 +2095,12 @@ hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, c
   return hipGetLastError();
 }
 
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid,
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid, int bits,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_light, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
+  if (bits == kLightBitsBig)
+    hipLaunchKernelGGL(k_light<kLightBitsBig>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
+  else
+    hipLaunchKernelGGL(k_light<kLightBits>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
   return hipGetLastError();
 }
 
@@ -2187,9 +2203,11 @@ int heavy_occupancy(int threads, size_t lds_bytes, bool hbm_bitmap) {
   return occ;
 }
 
-int light_occupancy() {
+int light_occupancy(int bits) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_light, kLightWaves * 64, 0) != hipSuccess) return 0;
-  return nb;
+  const hipError_t e = bits == kLightBitsBig
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_light<kLightBitsBig>, kLightWaves * 64, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_light<kLightBits>, kLightWaves * 64, 0);
+  return e == hipSuccess ? nb : 0;
 }
 }  // namespace gsofa
