@@ -370,6 +370,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   if (opt.split_blocks != -1) {
     std::vector<int32_t> cut;
     for (int b = 0; b < nmats_; ++b) {
+      // automatic: only matrices of at least a quarter of the set's rows, whose longest item can
+      // bound the launch; in a set of many small matrices the launch is bound by total work,
+      // which a cut adds to (each part traverses its rows' subtrees afresh)
+      if (opt.split_blocks == 0 && (int64_t)ns_[b] * kCutShare < nrows_) continue;
       const int64_t c = block_cuts(*mats[b], opt_.chain_block, opt.split_blocks > 0 ? opt.split_blocks : kCutDivisor, cut);
       if (c == 0) continue;
       if (cut_all.empty()) cut_all.assign((size_t)nblk_total_, -1);

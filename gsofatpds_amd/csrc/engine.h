@@ -33,6 +33,9 @@ struct Prepared {
 // traversal of that subtree begins there), the largest such; -1 where none. Returns the count.
 int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std::vector<int32_t>& cut);
 constexpr int32_t kCutDivisor = 64;  // the automatic divisor
+// Automatic cuts apply to matrices holding at least 1 / kCutShare of the set's rows. Measured on
+// 256 matrices of 10,000-200,000 rows in one set: cutting every member 149 -> 160 ms per run.
+constexpr int32_t kCutShare = 4;
 
 // Whether the workgroup path runs both directions of a block together (Options::pair_directions).
 bool pair_directions(const Prepared& P, int32_t mode);
@@ -55,7 +58,9 @@ struct Options {
   int32_t cu_share = 0;         // size persistent grids for this many CUs (0 = all): lets
                                 // concurrent sets on other streams overlap on the device
   int32_t split_blocks = 0;     // cut blocks at large separator starts (block_cuts): 0 automatic
-                                // (kCutDivisor), k > 0 subtrees of at least n / k rows, -1 never
+                                // (kCutDivisor, matrices of at least 1 / kCutShare of the set's
+                                // rows), k > 0 every matrix with subtrees of at least n / k rows,
+                                // -1 never
 };
 
 struct MatResult {

@@ -74,10 +74,12 @@ struct WorkState {
   unsigned long long pad;
 };
 
-// Item encoding for the escalation list: part of a cut block (bit 63), matrix index (bits
-// 33..62), direction, block of chain_block rows.
-__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t block, uint32_t part = 0) {
-  return ((uint64_t)part << 63) | ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)block;
+// Item encoding for the escalation list: rows before a cut (bit 63), matrix index (bits
+// 33..62), direction (bit 32), a row v (bits 0..31). The item runs rows v to the end of v's block
+// of chain_block rows (v is the block's first row, or a cut), or, with bit 63, the rows of v's
+// block before v.
+__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t row, uint32_t before_cut) {
+  return ((uint64_t)before_cut << 63) | ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)row;
 }
 
 }  // namespace gsofa

@@ -841,6 +841,14 @@ __device__ __forceinline__ bool heavy_slot(const MatDev* mats, int64_t nblk_tota
   return blk_flag[g] != 0;
 }
 
+// The row where block b of matrix M is cut into two workgroup-path items, or -1.
+__device__ __forceinline__ int32_t block_cut(const MatDev& M, int32_t b) {
+  if (!M.split) return -1;
+  const int32_t cut = M.split[b];
+  const int64_t r0 = (int64_t)b * M.chain_block;
+  return cut > r0 && cut < min((int64_t)M.n, r0 + M.chain_block) ? cut : -1;
+}
+
 // ------------------------------------------------------------------------------------------
 __global__ void k_heavy_hist(const MatDev* __restrict__ mats, int64_t nblk_total, int nmats,
                              const int64_t* __restrict__ blkq_base, const uint32_t* __restrict__ blk_flag,
@@ -851,7 +859,7 @@ __global__ void k_heavy_hist(const MatDev* __restrict__ mats, int64_t nblk_total
     uint32_t dir;
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
-    const int parts = (mats[m].split && mats[m].split[b] >= 0) ? 2 : 1;
+    const int parts = block_cut(mats[m], b) >= 0 ? 2 : 1;
     atomicAdd(&bucket[nb_max - 1 - b], parts);
     atomicAdd(&ws->heavy_count, (unsigned long long)parts);
   }
@@ -866,14 +874,15 @@ __global__ void k_heavy_scatter(const MatDev* __restrict__ mats, int64_t nblk_to
     uint32_t dir;
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
-    if (mats[m].split && mats[m].split[b] >= 0) {
+    const int32_t cut = block_cut(mats[m], b);
+    if (cut >= 0) {
       // a cut block: the part from the cut on (the larger subtree) goes first
       const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 2);
-      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b, 1u);
-      heavy_items[pos + 1] = encode_item((uint32_t)m, dir, (uint32_t)b, 0u);
+      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)cut, 0u);
+      heavy_items[pos + 1] = encode_item((uint32_t)m, dir, (uint32_t)cut, 1u);
     } else {
       const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
-      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)b);
+      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)(b * mats[m].chain_block), 0u);
     }
   }
 }
@@ -1068,22 +1077,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const uint64_t item = heavy_items[idx];
     const int mb = (int)((item >> 33) & 0x3fffffffu);
     const int item_dir = (int)((item >> 32) & 1);
-    const int32_t blk = (int32_t)(item & 0xffffffffu);
-    const int part = (int)(item >> 63);
+    const int32_t v = (int32_t)(item & 0xffffffffu);
+    const int before_cut = (int)(item >> 63);
     const MatDev& M = mats[mb];
     const bool paired = kPairedAny && M.paired != 0;
     const int ndir = paired ? 2 : 1;
     const int32_t n = M.n;
     const int32_t bw = (n + 31) >> 5;
-    int32_t r0 = blk * M.chain_block;
-    int32_t r1 = min(n, r0 + M.chain_block);
-    if (M.split) {  // a cut block: part 0 runs the rows before the cut, part 1 the rest
-      const int32_t cut = M.split[blk];
-      if (cut > r0 && cut < r1) {
-        if (part) r0 = cut;
-        else r1 = cut;
-      }
-    }
+    // rows [r0, r1): from v to the end of its block, or, before a cut, from the block's start to
+    // v (decoded from the item alone: a table read here costs the traversal registers)
+    const int32_t vb = v - v % M.chain_block;
+    const int32_t r0 = before_cut ? vb : v;
+    const int32_t r1 = before_cut ? v : min(n, vb + M.chain_block);
     const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
     // The leader's state of this item, per direction (index = direction; a plain item uses its

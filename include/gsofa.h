@@ -155,7 +155,10 @@ typedef struct gsofa_options {
                                a large subtree (its elimination subtree at least 1.5x its
                                predecessor's and n / k rows) runs as two items, so two
                                traversals of whole subtrees do not queue in one workgroup:
-                               0 (default) k = 64, k > 0 that k, -1 never                      */
+                               0 (default) k = 64 for matrices of at least a quarter of the
+                               set's rows (a plan's matrix; not the members of a set of many,
+                               where the extra traversals cost more than they shorten), k > 0
+                               that k for every matrix, -1 never                               */
 } gsofa_options;
 
 typedef struct gsofa_result {
