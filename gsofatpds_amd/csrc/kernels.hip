@@ -901,6 +901,16 @@ __device__ __forceinline__ void qput(int32_t* l, int32_t* g, int32_t qcap, int32
   else gstore(g + (e - qcap), v);
 }
 
+// Writes the entries of rc at the set bits of mask (one lane's new vertices) through put at
+// pos, pos + 1, ... A slot no lane of the wave sets (a zero bit of any, the OR of mask over the
+// wave) costs a scalar branch instead of a predicated write. Wave-uniform call.
+template <typename Put>
+__device__ __forceinline__ void put_row_entries(const RowCursor& rc, uint32_t mask, uint32_t any, int32_t pos, Put put) {
+#pragma unroll
+  for (int j = 0; j < kEll; ++j)
+    if (((any >> j) & 1u) && ((mask >> j) & 1u)) put(pos++, rc.e[j]);
+}
+
 // A row traversed by the bitmap path: W_R, input degree, |R|, |U| and its staged U-list length.
 struct RowFacts {
   uint64_t wr, deg, r;
@@ -1018,6 +1028,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ unsigned long long s_escm[2];      // rows w0 .. w0 + 63 escalated, as bits (per direction)
   __shared__ HeavyCtl s_ctl;
   __shared__ LeaderState s_ls;
+  // The slots some lane of the wave sets (push loops skip the others). Sets with paired items
+  // (unstructured rows, which fill most slots) keep every slot: measured faster there.
+  auto row_any = [](uint32_t mask) -> uint32_t {
+    if constexpr (kPairedAny) return ~0u;
+    else return wave_or_u32(mask);
+  };
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
@@ -1517,20 +1533,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   {
                     const uint32_t c = __popc(mask_r);
                     const uint32_t incl = wave_incl_scan_u32(c);
-                    int32_t pos = nxt_n + (int32_t)(incl - c);
-#pragma unroll
-                    for (int j = 0; j < kEll; ++j)
-                      if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
-                    nxt_n += __builtin_amdgcn_readlane((int)incl, 63);
+                    const int32_t pos = nxt_n + (int32_t)(incl - c);
+                    const int32_t wend = nxt_n + __builtin_amdgcn_readlane((int)incl, 63);
+                    const uint32_t any = row_any(mask_r);
+                    if (!kPairedAny && wend <= qcap)  // the whole wave's range is in LDS
+                      put_row_entries(rc, mask_r, any, pos, [&](int32_t i, int32_t x) { nxt_l[i] = x; });
+                    else
+                      put_row_entries(rc, mask_r, any, pos, [&](int32_t i, int32_t x) { qput(nxt_l, nxt_g, qcap, i, x); });
+                    nxt_n = wend;
                   }
                   u_new += __popc(mask_u);
                   if (want_list) {
                     const uint32_t c = __popc(mask_u);
                     const uint32_t incl = wave_incl_scan_u32(c);
-                    uint32_t pos = ubase + ust + (incl - c);
-#pragma unroll
-                    for (int j = 0; j < kEll; ++j)
-                      if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
+                    const uint32_t pos = ubase + ust + (incl - c);
+                    put_row_entries(rc, mask_u, row_any(mask_u), (int32_t)pos,
+                                    [&](int32_t i, int32_t x) { uput((uint32_t)i, x); });
                     ust += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
                   }
                   // CSR tails of long rows: one entry per lane per step
@@ -1901,10 +1919,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   if (tot_w != 0) {
                     int32_t basepos = 0;
                     if (lane == 63) basepos = atomicAdd(nxt_cnt, (int32_t)tot_w);
-                    int32_t pos = __builtin_amdgcn_readlane(basepos, 63) + (int32_t)(incl - c);
-#pragma unroll
-                    for (int j = 0; j < kEll; ++j)
-                      if ((mask_r >> j) & 1u) qput(nxt_l, nxt_g, qcap, pos++, rc.e[j]);
+                    const int32_t wbase = __builtin_amdgcn_readlane(basepos, 63);
+                    const int32_t pos = wbase + (int32_t)(incl - c);
+                    const uint32_t any = row_any(mask_r);
+                    if (!kPairedAny && wbase + (int32_t)tot_w <= qcap)  // the whole wave's range is in LDS
+                      put_row_entries(rc, mask_r, any, pos, [&](int32_t i, int32_t x) { nxt_l[i] = x; });
+                    else
+                      put_row_entries(rc, mask_r, any, pos, [&](int32_t i, int32_t x) { qput(nxt_l, nxt_g, qcap, i, x); });
                   }
                 }
                 p_un += __popc(mask_u);
@@ -1915,10 +1936,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   if (tot_w != 0) {
                     uint32_t basepos = 0;
                     if (lane == 63) basepos = atomicAdd(&s_ust, tot_w);
-                    uint32_t pos = cubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
-#pragma unroll
-                    for (int j = 0; j < kEll; ++j)
-                      if ((mask_u >> j) & 1u) uput(pos++, rc.e[j]);
+                    const uint32_t pos = cubase + (uint32_t)__builtin_amdgcn_readlane((int)basepos, 63) + (incl - c);
+                    put_row_entries(rc, mask_u, row_any(mask_u), (int32_t)pos,
+                                    [&](int32_t i, int32_t x) { uput((uint32_t)i, x); });
                   }
                 }
                 // the CSR tail of a long row, one entry per lane per step
