@@ -31,6 +31,7 @@ Options to_options(const gsofa_options* o) {
   r.autotune = o->autotune ? 1 : 0;
   r.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
   r.split_blocks = o->split_blocks < 0 ? -1 : o->split_blocks;
+  r.symmetric = o->symmetric < 0 ? -1 : 0;
   return r;
 }
 

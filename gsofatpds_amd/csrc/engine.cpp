@@ -42,8 +42,20 @@ int prepare(const gsofa_csr* a, Prepared& out) {
     out.dup_u.swap(dup);
     out.dup_all.swap(dup_all);
   }
+  // structural symmetry of the de-duplicated pattern (rows are sorted: one search per entry)
+  out.symmetric = true;
+  for (int32_t v = 0; v < n && out.symmetric; ++v)
+    for (int32_t k = out.rowptr[v]; k < out.rowptr[v + 1]; ++k) {
+      const int32_t c = out.adj[k];
+      if (c != v && !std::binary_search(out.adj.begin() + out.rowptr[c], out.adj.begin() + out.rowptr[c + 1], v)) {
+        out.symmetric = false;
+        break;
+      }
+    }
   return GSOFA_OK;
 }
+
+bool mirrored(const Prepared& P, const Options& opt) { return opt.symmetric != -1 && P.symmetric; }
 
 // Elimination tree of A + A^T by Liu's algorithm with path compression: every pair {k, i}, k < i,
 // of A's off-diagonal entries is taken at step i. Then subtree sizes.
@@ -289,10 +301,12 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.esc), q.esc_words, 0, s), "clear escalation flags");
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.blk_flag), 2 * q.nblk_total, 0, s), "clear block flags");
   GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.ell_a, s), "ell A");
-  GSOFA_SEQ(launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor,
-                             q.adj_t, q.block_sums, q.total, s),
-            "transpose");
-  GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
+  if (!q.mirror_all) {  // A^T: traversed from every matrix that is not mirrored
+    GSOFA_SEQ(launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor,
+                               q.adj_t, q.block_sums, q.total, s),
+              "transpose");
+    GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
+  }
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
   GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built
@@ -303,6 +317,7 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
   GSOFA_SEQ(launch_heavy(q.cfg.heavy_threads, q.cfg.heavy_grid, q.mats, q.ws, q.heavy_items, q.cfg.bw_max, q.cfg.qcap,
                          q.n_max, q.paired_any, q.spill, q.hbm_bm, q.cfg.ucap, s),
             "k_heavy");
+  if (q.mirror_any) GSOFA_SEQ(launch_mirror(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_mirror");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[3], s), "hipEventRecord");
   GSOFA_SEQ(launch_supernodes(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_supernodes");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[4], s), "hipEventRecord");
@@ -329,6 +344,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   blk_base_h_.assign(nmats_ + 1, 0);
   n_max_ = 0;
   bool any_dup = false;
+  mirror_.assign((size_t)nmats_, 0);
   for (int b = 0; b < nmats_; ++b) {
     const Prepared& P = *mats[b];
     ns_[b] = P.n;
@@ -337,7 +353,14 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     nnz_base_h_[b + 1] = nnz_base_h_[b] + (int64_t)P.adj.size();
     blk_base_h_[b + 1] = blk_base_h_[b] + ((int64_t)P.n + 255) / 256;
     any_dup |= !P.dup_u.empty();
-    paired_any_ |= pair_directions(P, opt.pair_directions) ? 1 : 0;
+    mirror_[b] = mirrored(P, opt) ? 1 : 0;
+    paired_any_ |= (!mirror_[b] && pair_directions(P, opt.pair_directions)) ? 1 : 0;
+  }
+  mirror_any_ = 0;
+  mirror_all_ = 1;
+  for (int b = 0; b < nmats_; ++b) {
+    mirror_any_ |= mirror_[b];
+    mirror_all_ &= mirror_[b];
   }
   nrows_ = row_base_h_[nmats_];
   nnz_ = nnz_base_h_[nmats_];
@@ -382,8 +405,12 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
       ncut += c;
     }
   }
-  for (int b = 0; b < nmats_; ++b)
-    item_base_h_[b + 1] = item_base_h_[b] + 2 * (((int64_t)ns_[b] + light_block_ - 1) / light_block_);
+  for (int b = 0; b < nmats_; ++b) {
+    // wave-path items: two per block (one per direction), or for a mirrored matrix one per block
+    // rounded up to even (the next matrix's items keep the parity the sibling exchange pairs by)
+    const int64_t nb = ((int64_t)ns_[b] + light_block_ - 1) / light_block_;
+    item_base_h_[b + 1] = item_base_h_[b] + (mirror_[b] ? nb + (nb & 1) : 2 * nb);
+  }
   nitems_ = item_base_h_[nmats_];
 
   // ---- allocations ----
@@ -489,7 +516,8 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.blk_flag[1] = blk_flag_ + nblk_total_ + blkq_base_h_[b];
     M.split = has_cut[b] ? cut_ + blkq_base_h_[b] : nullptr;
     M.chain_block = opt_.chain_block;
-    M.paired = pair_directions(*mats[b], opt_.pair_directions) ? 1 : 0;
+    M.sym = mirror_[b];
+    M.paired = (!mirror_[b] && pair_directions(*mats[b], opt_.pair_directions)) ? 1 : 0;
     M.light_block = light_block_;
     M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall
                     : cfg_.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
@@ -546,6 +574,8 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   q.nb_max = nb_max_;
   q.n_max = n_max_;
   q.paired_any = paired_any_;
+  q.mirror_any = mirror_any_;
+  q.mirror_all = mirror_all_;
   q.cfg = cfg_;
   q.rowptr_a = rowptr_a_;
   q.adj_a = adj_a_;
@@ -635,7 +665,7 @@ int DeviceSet::size_pools_from_counts() {
       up[pb + v] = su;
       lp[pb + v] = sl;
       su += cu[rb + v];
-      sl += cl[rb + v];
+      if (!mirror_[b]) sl += cl[rb + v];  // a mirrored matrix's L lists are its U lists
     }
     up[pb + ns_[b]] = su;
     lp[pb + ns_[b]] = sl;
@@ -643,6 +673,7 @@ int DeviceSet::size_pools_from_counts() {
     pool_base_l_[b + 1] = pool_base_l_[b] + sl;
     u_ptr_h_[b].assign(up.begin() + pb, up.begin() + pb + ns_[b] + 1);
     l_ptr_h_[b].assign(lp.begin() + pb, lp.begin() + pb + ns_[b] + 1);
+    if (mirror_[b]) l_ptr_h_[b] = u_ptr_h_[b];
   }
   if (!list_ptr_u_) {
     GSOFA_TRY(alloc(list_ptr_u_, (size_t)nptr, "U offsets"));
@@ -675,7 +706,11 @@ int DeviceSet::read_structure(int mat, std::vector<int64_t>& u_ptr, std::vector<
   u_idx.resize((size_t)nu);
   l_idx.resize((size_t)nl);
   if (nu) GSOFA_TRY(download(u_idx.data(), pool_u_ + pool_base_u_[mat], (size_t)nu * 4, "read U"));
-  if (nl) GSOFA_TRY(download(l_idx.data(), pool_l_ + pool_base_l_[mat], (size_t)nl * 4, "read L"));
+  if (mirror_[mat]) {
+    l_idx = u_idx;  // L's column v is U's row v
+  } else if (nl) {
+    GSOFA_TRY(download(l_idx.data(), pool_l_ + pool_base_l_[mat], (size_t)nl * 4, "read L"));
+  }
   const int32_t n = ns_[mat];
   for (int32_t v = 0; v < n; ++v) {
     std::sort(u_idx.begin() + u_ptr[v], u_idx.begin() + u_ptr[v + 1]);

@@ -22,6 +22,7 @@ struct Prepared {
   std::vector<int32_t> dup_u;   // empty when the input had no duplicate upper entries
   std::vector<int32_t> dup_all; // input row length minus de-duplicated length; empty if none
   int64_t hold_prev = 0;        // rows s >= 1 whose row holds s-1
+  bool symmetric = false;       // every off-diagonal (i, j) has its (j, i)
   // Subtree sizes of the elimination tree of A + A^T (|T[s]| bounds |R_s| + 1), computed on
   // first use (block cuts) and kept.
   mutable std::vector<int32_t> subtree;
@@ -39,6 +40,9 @@ constexpr int32_t kCutShare = 4;
 
 // Whether the workgroup path runs both directions of a block together (Options::pair_directions).
 bool pair_directions(const Prepared& P, int32_t mode);
+struct Options;
+// Whether a matrix is traversed from A only, with L mirrored from U (Options::symmetric).
+bool mirrored(const Prepared& P, const Options& opt);
 
 int prepare(const gsofa_csr* a, Prepared& out);
 
@@ -61,6 +65,8 @@ struct Options {
                                 // (kCutDivisor, matrices of at least 1 / kCutShare of the set's
                                 // rows), k > 0 every matrix with subtrees of at least n / k rows,
                                 // -1 never
+  int32_t symmetric = 0;        // 0 traverse structurally symmetric patterns once (L mirrored from
+                                // U), -1 always both directions
 };
 
 struct MatResult {
@@ -100,6 +106,8 @@ struct Sequence {
   int nmats = 0;
   int32_t nb_max = 0, n_max = 0;
   int32_t paired_any = 0;
+  int32_t mirror_any = 0;      // some matrix is mirrored (k_mirror runs)
+  int32_t mirror_all = 0;      // every matrix is mirrored (A^T is not built)
   LaunchConfig cfg;
   int32_t *rowptr_a = nullptr, *adj_a = nullptr, *ell_a = nullptr, *col_cnt = nullptr, *rowptr_t = nullptr;
   int32_t *cursor = nullptr, *adj_t = nullptr, *ell_t = nullptr, *block_sums = nullptr, *total = nullptr;
@@ -186,6 +194,8 @@ class DeviceSet {
   std::vector<int64_t> blkq_base_h_;  // per matrix, first block of its chain blocks
   int64_t* blkq_base_d_ = nullptr;
   int32_t paired_any_ = 0;            // some matrix runs paired workgroup-path items
+  int32_t mirror_any_ = 0, mirror_all_ = 0;  // mirrored matrices (Options::symmetric)
+  std::vector<uint8_t> mirror_;       // per matrix
   int heavy_wgs_per_cu_ = 0;          // resident bitmap-path workgroups per CU
   int32_t light_block_ = kLightBlockMin;  // rows per wave-path item
   int32_t nb_max_ = 0;                // chain blocks of the largest matrix

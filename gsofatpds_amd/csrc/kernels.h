@@ -62,6 +62,8 @@ struct MatDev {
                                   // workgroup-path items, or -1 (null: no block is cut)
   int32_t chain_block;            // K: rows per workgroup-path item (1 disables chaining)
   int32_t paired;                 // 1: workgroup-path items run both directions of a block
+  int32_t sym;                    // 1: mirrored (structurally symmetric: direction 0 only, L's
+                                  // counts and totals copied from U's by k_mirror)
   int32_t light_block;            // rows per wave-path item
   int32_t light_limit;            // reached vertices before the wave path escalates a row
 };
@@ -134,6 +136,9 @@ constexpr int64_t heavy_region_ints(int64_t n_max, bool paired_any) {
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
                         uint32_t* hbm_bitmaps, int32_t ucap, hipStream_t s);
+// Mirrored matrices: L counts = U counts, fill and workgroup-path entries doubled, nnz_L = nnz_U
+// (one thread per row, blocks as for launch_supernodes).
+hipError_t launch_mirror(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 hipError_t launch_supernodes(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);
 constexpr int kLightThreads = 256;
 constexpr int kLightWaves = 4;  // wavefronts per wave-path workgroup

@@ -478,15 +478,18 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     }
     const MatDev& M = mats[mb];
     const int64_t local = k - item_base[mb];
-    const int dir = (int)(local & 1);
+    // a mirrored matrix runs direction 0 only, one item per block (its last item may be padding)
+    const bool sym = M.sym != 0;
+    const int dir = sym ? 0 : (int)(local & 1);
     const int32_t lb = M.light_block;
     const int limit = M.light_limit;  // reached vertices before escalation
     // Lowest rows first, so that their escalations are seen by the rows above (the escalation
     // rules below; measured 3 % faster than highest first on the 659,033-row stencil).
-    const int32_t blk = (int32_t)(local >> 1);
+    const int32_t blk = sym ? (int32_t)local : (int32_t)(local >> 1);
     const int32_t s0 = blk * lb, s1 = min(M.n, s0 + lb);
-    // the U list is kept where it is read: for the structure, and for direction 1's window bits
-    const bool track_u = M.list_ptr[dir] != nullptr || dir == 1;
+    // the U list is kept where it is read: for the structure, and for the window bits of L's
+    // columns (direction 1, or a mirrored matrix's direction 0: L's column s is U's row s)
+    const bool track_u = M.list_ptr[dir] != nullptr || dir == 1 || sym;
 
     // state carried from row s-1 (all wave-uniform; hash H holds X_{s-1} when carry_ok)
     bool carry_ok = false;     // H holds X_{s-1}
@@ -498,6 +501,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     int uq = 0;
     uint32_t u_prev = 0;       // |U_{s-1}|
     uint64_t wr_prev = 0, deg_prev = 0, r_prev = 0;
+    int self_st = 3;           // mirrored: is src in X_{src-1}? 1 in, 2 not in, 3 unknown
     for (int32_t src = s0; src < s1; ++src) {
       const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
       RowCursor rc;
@@ -515,9 +519,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       // otherwise (when X_{src-1} is at hand or src-1 escalated): from the sibling wave's row
       // src-1 of the other direction, or through an entry whose own row holds src-1
       if (!chained && src > s0 && (carry_ok || prev_esc)) {
-        int st = 3;
+        // Mirrored, the other direction is this one: its own row src-1 answers (src in U_{src-1}).
+        int st = sym ? self_st : 3;
         const unsigned long long want = ((unsigned long long)(k ^ 1) << 32) | (uint32_t)src;
-        for (int spin = 0; spin < (1 << 16); ++spin) {  // bounded: an unknown answer is also correct
+        for (int spin = 0; !sym && spin < (1 << 16); ++spin) {  // bounded: an unknown answer is also correct
           const unsigned long long p = __hip_atomic_load(&s_pub[sib], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (p >= want) {
             // Published row src-1 of this block (or went past it). Its ring slot is rewritten when
@@ -768,7 +773,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           } else {
             t_nnz_l += U;
           }
-          if (dir == 1) {
+          if (dir == 1 || sym) {
             uint32_t win[4] = {0, 0, 0, 0};
             for (int i = lane; i < uq; i += 64) window_bit(win, src, Q[kLightQ - 1 - i], chunk_end);
             const uint32_t w0 = wave_or_u32(win[0]), w1 = wave_or_u32(win[1]);
@@ -804,7 +809,8 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       // publish row src for the sibling wave: is src+1 in X_src?
       {
         const uint8_t st = esc ? 3 : (src + 1 < s1 && hash_has<kBits>(H, src + 1)) ? 1 : 2;
-        if (lane == 0) {
+        self_st = st;
+        if (!sym && lane == 0) {
           // the previous row's publication is visible before its ring slot's next use
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           s_nxt[wave][src & 7] = st;
@@ -1255,6 +1261,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               t = t1;
               fi = kSavedRing;
               chained = hold && (lds_t1 || slot1 >= 0);
+              if (!chained && M.sym && t1 == src - 1 && (lds_t1 || slot1 >= 0)) {
+                // Mirrored (A^T = A): s-1 is in R_s exactly when s is in U_{s-1}, a bit of X_{s-1}
+                const uint32_t xw = lds_t1 ? bm_get(src >> 5) : gload(gslot(d, slot1) + (src >> 5));
+                chained = ((xw >> (src & 31)) & 1u) != 0;
+              }
               if (kPairedAny && paired && !chained) {
                 // ---- paired: early tests of the saved rows c of this direction whose other
                 // direction is saved too. An entry u of row s with u == c, or u < c in R'_c (u
@@ -1641,7 +1652,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
               // may chain from this one across wave-path rows, and the LDS copy will not survive
               // the other direction); plain, when the next row is escalated and does not hold
               // this one (only an import can use it).
-              const bool save = paired ? next_in : next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+              // Mirrored: never (s+1 chains from the LDS copy when it can, and otherwise s is not in
+              // R_{s+1}, so no import takes X_s).
+              const bool save = paired ? next_in
+                                       : !M.sym && next_in && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
               const int wslot = s_ls.wp[d];  // the ring slot this row's save takes
               if (lane == 0) {
                 const RowFacts f{wr, dg0, rr, U, want_list ? kept : 0u};
@@ -1663,7 +1677,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   tot.nnz_u += U;
                 } else {
                   tot.nnz_l += U;
-                  // window bits straight from the bitmap: rows src+1 .. src+128 of this column's chunk
+                }
+                if (d == 1 || M.sym) {
+                  // window bits straight from the bitmap: rows src+1 .. src+128 of this column's
+                  // chunk (a mirrored matrix: L's column src is U's row src)
                   uint32_t win[4];
                   for (int q = 0; q < 4; ++q) {
                     const int32_t lo = src + 1 + 32 * q;  // first row of this window word
@@ -1693,7 +1710,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             if (phase == 4) {
               // ---- bookkeeping after the row (and its save) ----
               const bool saved = paired ? src + 1 < r1
-                                        : src + 1 < r1 && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
+                                        : !M.sym && src + 1 < r1 && s_esc[d][li + 1] != 0 && s_hold[d][li + 1] == 0;
               if (lane == 0) {
                 if (saved) {
                   const int wslot = s_ls.wp[d];
@@ -2077,6 +2094,25 @@ __global__ __launch_bounds__(256) void k_supernodes(const MatDev* __restrict__ m
   if (threadIdx.x == 0 && s_sum) atomicAdd(&M.acc[kAccNsup], (unsigned long long)s_sum);
 }
 
+// k_mirror: a mirrored matrix was traversed from A only. From source s, A^T reaches exactly what
+// A reaches (the pattern is its own transpose), so L's column s holds U's row s: its count is
+// copied, the fill and the workgroup path's entries (both triangles) double, nnz_L = nnz_U.
+// Runs after both traversal kernels, before k_supernodes.
+__global__ __launch_bounds__(256) void k_mirror(const MatDev* __restrict__ mats, int nmats,
+                                                const int64_t* __restrict__ blk_base) {
+  const int mb = find_segment(blk_base, nmats, (int64_t)blockIdx.x);
+  const MatDev& M = mats[mb];
+  if (!M.sym) return;
+  const int64_t r = ((int64_t)blockIdx.x - blk_base[mb]) * 256 + threadIdx.x;
+  if (r < M.n) M.cnt[1][r] = M.cnt[0][r];
+  if ((int64_t)blockIdx.x == blk_base[mb] && threadIdx.x == 0) {
+    unsigned long long* acc = M.acc;
+    acc[kAccFill] *= 2;
+    acc[kAccNnzL] = acc[kAccNnzU];
+    acc[kAccHeavyNnz] *= 2;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launchers (the engine calls these; kernels stay private to this file)
 // ------------------------------------------------------------------------------------------
@@ -2085,6 +2121,12 @@ static int grid_for(int64_t work, int block, int cap) {
   if (g < 1) g = 1;
   if (g > cap) g = cap;
   return (int)g;
+}
+
+hipError_t launch_mirror(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mirror, dim3((unsigned)nblocks), dim3(256), 0, s, mats, nmats, blk_base);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t s) {

@@ -159,6 +159,12 @@ typedef struct gsofa_options {
                                set's rows (a plan's matrix; not the members of a set of many,
                                where the extra traversals cost more than they shorten), k > 0
                                that k for every matrix, -1 never                               */
+  int32_t symmetric;        /* 0 (default): a structurally symmetric pattern (after duplicate
+                               removal, every off-diagonal (i, j) has its (j, i)) is traversed
+                               once, from A: the traversal from source j over A^T then reaches
+                               exactly what the one over A reaches, so L's column j is U's row j
+                               and the L counts, lists and totals are copied from U's.
+                               -1: traverse A and A^T for every pattern                        */
 } gsofa_options;
 
 typedef struct gsofa_result {

@@ -63,9 +63,11 @@ def test_grid_4096():
 
 
 def test_reference_dataset():
+    # structurally symmetric: traversed from A only by default, from A and A^T with symmetric=-1
     r = check(load_ds(), S=128, label="ds_11948")
     assert r.fill_count == 980136
     check(load_ds(), S=1024, structure=False, label="ds_11948")
+    check(load_ds(), S=128, label="ds_11948 both directions", symmetric=-1)
 
 
 @pytest.mark.parametrize("n,density,seed", [(7, 0.3, 1), (100, 0.05, 2), (257, 0.02, 3), (1000, 0.004, 4),
@@ -153,7 +155,8 @@ def test_paired_directions_forced(chain_block):
 
 @pytest.mark.parametrize("pair", [0, 1])
 def test_paired_directions_reference_dataset(pair):
-    check(load_ds(), S=128, label=f"ds_11948 pair {pair}", pair_directions=pair)
+    # both directions (a mirrored matrix has one direction and no pairs)
+    check(load_ds(), S=128, label=f"ds_11948 pair {pair}", pair_directions=pair, symmetric=-1)
 
 
 @pytest.mark.parametrize("member", [7, 11, 1])
@@ -495,6 +498,83 @@ def test_block_cuts_change_nothing(case):
                         oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
                         g.fnv1a_array(plan.supernode_flags())))
     assert out[0] == out[1]
+
+
+def _symmetric_case(case):
+    if case == "grid2d_nd_192":
+        return g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=192, ny=192, order=g.ORDER_ND, seed=3), {}, True
+    if case == "grid3d_14_chain3":
+        return (g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=14, ny=14, nz=14, order=g.ORDER_ND, seed=8),
+                {"chain_block": 3}, True)
+    if case == "grid2d_natural_64":
+        return g.generate(kind=g.GEN_STENCIL, dims=2, points=5, nx=64, ny=64, order=g.ORDER_NATURAL, seed=2), {}, True
+    if case == "random_symmetric_2000":
+        rng = np.random.default_rng(21)
+        D = rng.random((2000, 2000)) < 0.002
+        D = D | D.T
+        np.fill_diagonal(D, rng.random(2000) < 0.8)  # some diagonals missing
+        return g.Pattern.from_dense(D), {}, True
+    # 110,592 rows, symmetric (the workload of that size drops entries of one triangle)
+    return g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=48, ny=48, nz=48, order=g.ORDER_ND, seed=4), {}, False
+
+
+@pytest.mark.parametrize("case", ["grid2d_nd_192", "grid3d_14_chain3", "grid2d_natural_64", "random_symmetric_2000",
+                                  "grid3d_48_symmetric"])
+def test_mirrored_symmetric_patterns(case):
+    """A structurally symmetric pattern is traversed from A only, L mirrored from U: the oracle's
+    results (where it is cheap) and exactly those of a run over both A and A^T, with less work
+    executed."""
+    p, opt, with_oracle = _symmetric_case(case)
+    say(f"[mirror] {case} n={p.n}")
+    if with_oracle:
+        check(p, label=f"{case} mirrored", **opt)
+    out, work = [], []
+    for sym in (0, -1):
+        with g.Plan(p, structure=1, symmetric=sym, **opt) as plan:
+            plan.run()
+            r = plan.result()
+            up, ui, lp, li = plan.structure()
+            out.append(((r.fill_count, r.nnz_F, r.nnz_L, r.nnz_U, r.edge_checks, r.reach, r.supernodes_counted),
+                        oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
+                        g.fnv1a_array(plan.supernode_flags())))
+            w = plan.executed_work()
+            work.append(w["light_edges"] + w["heavy_edges"])
+    assert out[0] == out[1]
+    # one traversal per source instead of two (the executed work varies a little with chaining)
+    assert work[0] < 0.7 * work[1], work
+
+
+def test_mirrored_and_unsymmetric_members_in_one_batch():
+    """A set mixing symmetric members (mirrored) and unsymmetric ones (both directions, some
+    paired): every member as in a set where nothing is mirrored, and as its own plan."""
+    rng = np.random.default_rng(33)
+    pats = []
+    for k in range(7):
+        if k % 2 == 0:
+            pats.append(g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=40 + 13 * k, ny=37 + 11 * k,
+                                   order=g.ORDER_ND, seed=k))
+        else:
+            n = 700 + 300 * k
+            D = rng.random((n, n)) < 3.0 / n
+            np.fill_diagonal(D, True)
+            pats.append(g.Pattern.from_dense(D))
+    say(f"[mirror batch] n={[p.n for p in pats]}")
+    keys = ("fill_count", "nnz_F", "nnz_L", "nnz_U", "edge_checks", "reach", "supernodes")
+    res = []
+    for sym in (0, -1):
+        b = g.Batch(pats, symmetric=sym)
+        b.run().sync()
+        res.append([tuple(getattr(b.result(i), k) for k in keys) + (g.fnv1a_array(b.nzu(i)),
+                                                                    g.fnv1a_array(b.supernode_flags(i)))
+                    for i in range(len(pats))])
+        b.close()
+    assert res[0] == res[1]
+    for i, p in enumerate(pats):
+        with g.Plan(p) as plan:
+            plan.run()
+            r = plan.result()
+            assert tuple(getattr(r, k) for k in keys) + (g.fnv1a_array(plan.nzu()),
+                                                          g.fnv1a_array(plan.supernode_flags())) == res[0][i]
 
 
 def test_full_size_structure_properties():
