@@ -1633,17 +1633,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 const int64_t off = s_lp[d][li];
                 const bool lp_ok = s_lp[d][li + 1] - off == (int64_t)U;
                 int32_t* const dst = M.pool[d] + off;
-                for (uint32_t i0 = 0; i0 < len; i0 += 64) {
-                  const uint32_t i = i0 + lane;
-                  const int32_t e = i < len ? uget(i) : -1;
-                  const bool keep = e > src;
-                  const uint64_t mk = __ballot(keep);
-                  const uint32_t pos = kept + rank_below(mk);
-                  if (keep) {
-                    if (lp_ok && pos < U) gstore(dst + pos, e);
-                    uput(pos, e);
+                // 256 entries per round, all read before any is written: an entry moves to a
+                // position at or below its own, so a round never overwrites what it still reads
+                // and later rounds read only positions no round has written
+                for (uint32_t i0 = 0; i0 < len; i0 += 256) {
+                  int32_t e[4];
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) {
+                    const uint32_t i = i0 + 64 * q + lane;
+                    e[q] = i < len ? uget(i) : -1;
                   }
-                  kept += (uint32_t)__popcll(mk);
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) {
+                    const bool keep = e[q] > src;
+                    const uint64_t mk = __ballot(keep);
+                    const uint32_t pos = kept + rank_below(mk);
+                    if (keep) {
+                      if (lp_ok && pos < U) gstore(dst + pos, e[q]);
+                      uput(pos, e[q]);
+                    }
+                    kept += (uint32_t)__popcll(mk);
+                  }
                 }
                 list_ok = lp_ok && kept == U;
               }
