@@ -32,6 +32,7 @@ Options to_options(const gsofa_options* o) {
   r.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
   r.split_blocks = o->split_blocks < 0 ? -1 : o->split_blocks;
   r.symmetric = o->symmetric < 0 ? -1 : 0;
+  r.cu_share = o->cu_share > 0 ? o->cu_share : 0;
   return r;
 }
 

@@ -165,6 +165,9 @@ typedef struct gsofa_options {
                                exactly what the one over A reaches, so L's column j is U's row j
                                and the L counts, lists and totals are copied from U's.
                                -1: traverse A and A^T for every pattern                        */
+  int32_t cu_share;         /* 0 (default): persistent grids sized for every CU; k > 0: for k
+                               CUs, so that work the caller runs beside it on other CUs (a
+                               stream with a CU mask) keeps its share of the device            */
 } gsofa_options;
 
 typedef struct gsofa_result {

@@ -224,8 +224,9 @@ def test_tuning_configurations_are_identical():
     base = None
     for opt in (dict(), dict(heavy_threads=256), dict(heavy_threads=1024), dict(queue_capacity=256),
                 dict(light_wgs_per_cu=1, heavy_wgs_per_cu=1), dict(chain_block=1), dict(chain_block=7),
-                dict(chain_block=1024, queue_capacity=256), dict(pair_directions=1),
-                dict(pair_directions=1, chain_block=3, heavy_threads=256)):
+                dict(chain_block=1024, queue_capacity=256), dict(pair_directions=1, symmetric=-1),
+                dict(pair_directions=1, chain_block=3, heavy_threads=256, symmetric=-1), dict(symmetric=-1),
+                dict(cu_share=64), dict(cu_share=7, symmetric=-1)):
         say(f"[tuning] {opt}")
         with g.Plan(p, **opt) as plan:
             plan.run()
