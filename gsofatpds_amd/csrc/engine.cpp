@@ -285,6 +285,23 @@ int configure_launch(int32_t n_max, int64_t nrows, bool paired_any, const Option
   return GSOFA_OK;
 }
 
+int enqueue_preparation(const Sequence& q, hipStream_t s) {
+  auto hf = [](hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? GSOFA_ERR_OUT_OF_MEMORY : GSOFA_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e = launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.ell_a, s);
+  if (e != hipSuccess) return hf(e, "ell A");
+  if (!q.mirror_all) {  // A^T: traversed from every matrix that is not mirrored
+    e = launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor, q.adj_t,
+                         q.block_sums, q.total, s);
+    if (e != hipSuccess) return hf(e, "transpose");
+    e = launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s);
+    if (e != hipSuccess) return hf(e, "ell A^T");
+  }
+  return GSOFA_OK;
+}
+
 int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
   auto hf = [](hipError_t e, const char* what) {
     return fail(e == hipErrorOutOfMemory ? GSOFA_ERR_OUT_OF_MEMORY : GSOFA_ERR_HIP,
@@ -300,13 +317,7 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
   GSOFA_SEQ(launch_fill_u64(reinterpret_cast<unsigned long long*>(q.ws), 4, 0ull, s), "clear work state");
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.esc), q.esc_words, 0, s), "clear escalation flags");
   GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.blk_flag), 2 * q.nblk_total, 0, s), "clear block flags");
-  GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.ell_a, s), "ell A");
-  if (!q.mirror_all) {  // A^T: traversed from every matrix that is not mirrored
-    GSOFA_SEQ(launch_transpose(q.nrows, q.nmats, q.row_base, q.rowptr_a, q.adj_a, q.col_cnt, q.rowptr_t, q.cursor,
-                               q.adj_t, q.block_sums, q.total, s),
-              "transpose");
-    GSOFA_SEQ(launch_ell_build(q.nrows, q.nmats, q.row_base, q.rowptr_t, q.adj_t, q.ell_t, s), "ell A^T");
-  }
+  if (q.prepare) GSOFA_TRY(enqueue_preparation(q, s));
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
   GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built
@@ -523,7 +534,10 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
                     : cfg_.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
-  if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(table)");
+  // The pattern is fixed for the set's lifetime, and so are its device forms (ELL rows, A^T): built
+  // here once, like the upload, not on every run.
+  GSOFA_TRY(enqueue_preparation(sequence(), alloc_stream_));
+  GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(preparation)");
   return GSOFA_OK;
 }
 
@@ -564,6 +578,12 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
 
 int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   GSOFA_HIP(hipSetDevice(opt_.device), "hipSetDevice");
+  Sequence q = sequence();
+  q.prepare = 0;  // built once by create()
+  return enqueue_sequence(q, s, ev);
+}
+
+Sequence DeviceSet::sequence() const {
   Sequence q;
   q.nrows = nrows_;
   q.nitems = nitems_;
@@ -600,7 +620,7 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   q.row_base = row_base_d_;
   q.blk_base = blk_base_d_;
   q.blkq_base = blkq_base_d_;
-  return enqueue_sequence(q, s, ev);
+  return q;
 }
 
 int DeviceSet::read_results(std::vector<MatResult>& out) {

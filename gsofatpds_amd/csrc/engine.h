@@ -106,6 +106,7 @@ struct Sequence {
   int nmats = 0;
   int32_t nb_max = 0, n_max = 0;
   int32_t paired_any = 0;
+  int32_t prepare = 1;         // 0: the ELL rows and A^T are already built (enqueue_preparation)
   int32_t mirror_any = 0;      // some matrix is mirrored (k_mirror runs)
   int32_t mirror_all = 0;      // every matrix is mirrored (A^T is not built)
   LaunchConfig cfg;
@@ -121,9 +122,13 @@ struct Sequence {
   int64_t *item_base = nullptr, *row_base = nullptr, *blk_base = nullptr, *blkq_base = nullptr;
 };
 // Enqueues the whole run on `s` (kernels only: no allocation, no memset node, no host
-// synchronisation). ev (optional, 5 events): before preparation, after preparation, after the
-// wave path and the workgroup-path queue, after the workgroup path, after supernodes.
+// synchronisation). ev (optional, 5 events): before the run, after its clears (and the
+// preparation when q.prepare), after the wave path and the workgroup-path queue, after the
+// workgroup path, after supernodes.
 int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev);
+// The input's device forms: ELL rows of A, and A^T with its ELL rows unless every matrix is
+// mirrored. Part of enqueue_sequence when q.prepare is set.
+int enqueue_preparation(const Sequence& q, hipStream_t s);
 
 // Device state for one matrix (plain plan) or many (batch); all matrices share one launch of
 // each kernel.
@@ -138,6 +143,8 @@ class DeviceSet {
   // Enqueue the whole symbolic factorisation on `s` (no host synchronisation, no allocation,
   // no memset nodes: safe to capture in a graph).
   int enqueue(hipStream_t s, hipEvent_t* stage_events = nullptr);
+  // The launch sequence over this set's buffers (the preparation included).
+  Sequence sequence() const;
   // After the stream completed: read per-matrix totals.
   int read_results(std::vector<MatResult>& out);
   int read_counts(int mat, int32_t* cnt_u, int32_t* cnt_l, uint8_t* sup_flag, int32_t* nzu);

@@ -227,9 +227,10 @@ int gsofa_supernodal_from_structure(int32_t n, const uint8_t* flags, const int64
                                     const int64_t* l_colptr, const int32_t* l_row, int64_t* nsuper, int64_t* nnz_lsub,
                                     int64_t* nnz_usub, int32_t* xsup, int64_t* xlsub, int32_t* lsub, int64_t* xusub,
                                     int32_t* usub);
-/* Device time of the last run's stages, from events on the run's stream: ms[0] preparation
- * (ELL packing + transpose), ms[1] the wave (hash) path, ms[2] the workgroup (bitmap) path,
- * ms[3] supernode detection. No counterpart in the reference (its only timers are wtime()). */
+/* Device time of the last run's stages, from events on the run's stream: ms[0] the run's clears
+ * (the pattern's device forms, ELL rows and the transpose, are built once when the plan or batch
+ * is created), ms[1] the wave (hash) path, ms[2] the workgroup (bitmap) path, ms[3] supernode
+ * detection. No counterpart in the reference (its only timers are wtime()). */
 int gsofa_stage_times(gsofa_plan* plan, double* ms);
 /* The workgroup path's share of the last run: row-traversal edge checks, |R_i| sum, and strict
  * F entries (both directions). Used to price that kernel alone. */
