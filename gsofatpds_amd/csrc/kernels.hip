@@ -507,6 +507,13 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       RowCursor rc;
       // the source row, one ELL slot per lane (lanes 0..15); slot 15 <= -2 encodes a CSR tail
       const int32_t sm = lane < kEll ? gload(M.ell[dir] + (int64_t)src * kEll + lane) : kPad;
+      // the row's structure offsets, loaded now and used when it finishes (no round trip there);
+      // the 1,024-slot variant has no registers to spare and loads them at the finish
+      int64_t lp_off = 0, lp_end = 0;
+      if (kBits == kLightBitsBig && M.list_ptr[dir] != nullptr) {
+        lp_off = gload(M.list_ptr[dir] + src);
+        lp_end = gload(M.list_ptr[dir] + src + 1);
+      }
       // does row src hold src-1?
       bool has_prev_in_row = false;
       if (src > s0) {
@@ -782,8 +789,8 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
           }
           // structure: the U list at the back of Q
           if (M.list_ptr[dir] != nullptr) {
-            const int64_t off = gload(M.list_ptr[dir] + src);
-            const int64_t end = gload(M.list_ptr[dir] + src + 1);
+            const int64_t off = kBits == kLightBitsBig ? lp_off : gload(M.list_ptr[dir] + src);
+            const int64_t end = kBits == kLightBitsBig ? lp_end : gload(M.list_ptr[dir] + src + 1);
             if (end - off != (int64_t)U || uq != (int)U || qn + uq > kLightQ) {
               if (lane == 0) atomicOr(&M.acc[kAccOverflow], 1ull);
             } else {
@@ -1554,7 +1561,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     nxt_n = wend;
                   }
                   u_new += __popc(mask_u);
-                  if (want_list) {
+                  if (want_list && __ballot(mask_u != 0) != 0) {  // most levels find no U entry
                     const uint32_t c = __popc(mask_u);
                     const uint32_t incl = wave_incl_scan_u32(c);
                     const uint32_t pos = ubase + ust + (incl - c);
@@ -1956,7 +1963,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                   }
                 }
                 p_un += __popc(mask_u);
-                if (want_list) {
+                if (want_list && __ballot(mask_u != 0) != 0) {  // most passes find no U entry
                   const uint32_t c = __popc(mask_u);
                   const uint32_t incl = wave_incl_scan_u32(c);
                   const uint32_t tot_w = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
