@@ -1643,12 +1643,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 // 256 entries per round, all read before any is written: an entry moves to a
                 // position at or below its own, so a round never overwrites what it still reads
                 // and later rounds read only positions no round has written
+                int32_t* const ul = d ? ul1 : ul0;
                 for (uint32_t i0 = 0; i0 < len; i0 += 256) {
+                  // a round wholly inside the list's LDS part (the usual case) reads and writes
+                  // LDS without the per-entry test for the HBM part, so its reads overlap
+                  const bool in_lds = (int64_t)i0 + 256 <= (int64_t)ucap;
                   int32_t e[4];
+                  if (in_lds) {
 #pragma unroll
-                  for (int q = 0; q < 4; ++q) {
-                    const uint32_t i = i0 + 64 * q + lane;
-                    e[q] = i < len ? uget(i) : -1;
+                    for (int q = 0; q < 4; ++q) {
+                      const uint32_t i = i0 + 64 * q + lane;
+                      e[q] = i < len ? ul[i] : -1;
+                    }
+                  } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                      const uint32_t i = i0 + 64 * q + lane;
+                      e[q] = i < len ? uget(i) : -1;
+                    }
                   }
 #pragma unroll
                   for (int q = 0; q < 4; ++q) {
@@ -1657,7 +1669,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     const uint32_t pos = kept + rank_below(mk);
                     if (keep) {
                       if (lp_ok && pos < U) gstore(dst + pos, e[q]);
-                      uput(pos, e[q]);
+                      if (in_lds) ul[pos] = e[q];  // pos <= i0 + 255 < ucap
+                      else uput(pos, e[q]);
                     }
                     kept += (uint32_t)__popcll(mk);
                   }
