@@ -313,10 +313,10 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
     if (e_ != hipSuccess) return hf(e_, what);  \
   } while (0)
   if (ev) GSOFA_SEQ(hipEventRecord(ev[0], s), "hipEventRecord");
-  GSOFA_SEQ(launch_fill_u64(q.acc, (int64_t)q.nmats * kAccSlots, 0ull, s), "clear accumulators");
-  GSOFA_SEQ(launch_fill_u64(reinterpret_cast<unsigned long long*>(q.ws), 4, 0ull, s), "clear work state");
-  GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.esc), q.esc_words, 0, s), "clear escalation flags");
-  GSOFA_SEQ(launch_fill_i32(reinterpret_cast<int32_t*>(q.blk_flag), 2 * q.nblk_total, 0, s), "clear block flags");
+  GSOFA_SEQ(launch_clear_run(q.acc, (int64_t)q.nmats * kAccSlots, reinterpret_cast<unsigned long long*>(q.ws), 4,
+                             reinterpret_cast<int32_t*>(q.esc), q.esc_words, reinterpret_cast<int32_t*>(q.blk_flag),
+                             2 * q.nblk_total, q.bucket, q.nb_max, s),
+            "clear the run's state");
   if (q.prepare) GSOFA_TRY(enqueue_preparation(q, s));
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
   GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");

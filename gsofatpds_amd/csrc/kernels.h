@@ -92,6 +92,11 @@ __host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint
 
 namespace gsofa {
 hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t s);
+// A run's clears in one launch (all to zero): nacc u64 at acc, nws u64 at ws, nesc, nblk and
+// nbucket i32 words at esc, blk_flag and bucket.
+hipError_t launch_clear_run(unsigned long long* acc, int64_t nacc, unsigned long long* ws, int64_t nws, int32_t* esc,
+                            int64_t nesc, int32_t* blk_flag, int64_t nblk, int32_t* bucket, int64_t nbucket,
+                            hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s);
 hipError_t launch_ell_build(int64_t nrows, int nmats, const int64_t* row_base, const int32_t* rowptr,
                             const int32_t* adj, int32_t* ell, hipStream_t s);

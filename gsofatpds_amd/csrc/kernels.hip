@@ -142,6 +142,21 @@ __global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
     p[i] = v;
 }
 
+// A run's clears in one launch: accumulators and work state (u64), escalation and block flags
+// and the workgroup-path queue's buckets (i32 words), all to zero.
+__global__ void k_clear_run(unsigned long long* __restrict__ acc, int64_t nacc, unsigned long long* __restrict__ ws,
+                            int64_t nws, int32_t* __restrict__ esc, int64_t nesc, int32_t* __restrict__ blk_flag,
+                            int64_t nblk, int32_t* __restrict__ bucket, int64_t nbucket) {
+  const int64_t total = nacc + nws + nesc + nblk + nbucket;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < nacc) acc[i] = 0ull;
+    else if (i < nacc + nws) ws[i - nacc] = 0ull;
+    else if (i < nacc + nws + nesc) esc[i - nacc - nws] = 0;
+    else if (i < nacc + nws + nesc + nblk) blk_flag[i - nacc - nws - nesc] = 0;
+    else bucket[i - nacc - nws - nesc - nblk] = 0;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // layout: ELL-16 packing and the transpose
 // ------------------------------------------------------------------------------------------
@@ -2164,6 +2179,14 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t count, unsigned long l
   return hipGetLastError();
 }
 
+hipError_t launch_clear_run(unsigned long long* acc, int64_t nacc, unsigned long long* ws, int64_t nws, int32_t* esc,
+                            int64_t nesc, int32_t* blk_flag, int64_t nblk, int32_t* bucket, int64_t nbucket,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_run, dim3(grid_for(nacc + nws + nesc + nblk + nbucket, 256, 8192)), dim3(256), 0, s, acc,
+                     nacc, ws, nws, esc, nesc, blk_flag, nblk, bucket, nbucket);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t* p, int64_t count, int32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(count, 256, 8192)), dim3(256), 0, s, p, count, v);
   return hipGetLastError();
@@ -2206,9 +2229,16 @@ hipError_t launch_heavy_order(const MatDev* mats, int64_t nblk_total, int nmats,
                               int32_t* block_sums, int32_t* total, WorkState* ws, uint64_t* heavy_items, hipStream_t s) {
   const int64_t nslots = 2 * nblk_total;
   const int g = grid_for(nslots, 256, 8192);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nb_max, 256, 8192)), dim3(256), 0, s, bucket, (int64_t)nb_max, 0);
+  // bucket is zero already (the run's clears)
   hipLaunchKernelGGL(k_heavy_hist, dim3(g), dim3(256), 0, s, mats, nblk_total, nmats, blkq_base, blk_flag, nb_max, bucket,
                      ws);
+  if (nb_max <= 16 * kScanBlock) {
+    // one workgroup scans the buckets in place; they then serve as the scatter's cursors
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, total);
+    hipLaunchKernelGGL(k_heavy_scatter, dim3(g), dim3(256), 0, s, mats, nblk_total, nmats, blkq_base, blk_flag, nb_max,
+                       bucket, heavy_items);
+    return hipGetLastError();
+  }
   const int64_t per = (int64_t)kScanBlock * kScanItems;
   const int64_t nb = (nb_max + per - 1) / per;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, bucket, (int64_t)nb_max, block_sums);
