@@ -319,7 +319,7 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
             "clear the run's state");
   if (q.prepare) GSOFA_TRY(enqueue_preparation(q, s));
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
-  GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
+  GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.ws, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built
   GSOFA_SEQ(launch_heavy_order(q.mats, q.nblk_total, q.nmats, q.blkq_base, q.blk_flag, q.nb_max, q.bucket, q.cursor,
                                q.block_sums, q.total, q.ws, q.heavy_items, s),
@@ -630,6 +630,7 @@ int DeviceSet::read_results(std::vector<MatResult>& out) {
   for (int b = 0; b < nmats_; ++b) {
     const unsigned long long* a = acc.data() + (size_t)b * kAccSlots;
     if (a[kAccOverflow]) return fail(GSOFA_ERR_STATE, "structure offsets do not match this run's counts");
+    if (a[kAccScheduleError]) return fail(GSOFA_ERR_STATE, "wave path: a wave stopped waiting for its sibling's item");
     MatResult& r = out[b];
     r.fill = a[kAccFill];
     r.W = a[kAccW];

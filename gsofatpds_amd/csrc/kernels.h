@@ -37,6 +37,7 @@ enum : int {
   kAccLightExecEdges = 16,  // neighbour inspections done by k_light (escalated attempts included)
   kAccLightExecVerts = 17,  // rows expanded by k_light
   kAccLightExecOut = 18,    // F entries of the sources k_light finished
+  kAccScheduleError = 19,   // nonzero: a wave-path wave stopped waiting for its sibling's next item
   kAccSlots = 24
 };
 
@@ -70,7 +71,7 @@ struct MatDev {
 
 // Scheduler state of one launch set (plain matrix or batch).
 struct WorkState {
-  unsigned long long light_next;   // next item for the wave path
+  unsigned long long light_next;   // next pair of wave-path items
   unsigned long long heavy_count;  // items escalated to the workgroup path
   unsigned long long heavy_next;   // next escalated item
   unsigned long long pad;
@@ -106,7 +107,7 @@ hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, c
                             const int32_t* adj, int32_t* col_cnt, int32_t* rowptr_t, int32_t* cursor,
                             int32_t* adj_t, int32_t* block_sums, int32_t* total, hipStream_t s);
 // bits: kLightBits, or kLightBitsBig (a hash set of 2,048 slots and an escalation limit of 1,024)
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid, int bits,
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws, int grid, int bits,
                         hipStream_t s);
 // The escalated blocks (blk_flag, 2 * nblk_total slots) into heavy_items in descending block
 // order; sets ws->heavy_count. bucket has nb_max entries, cursor nb_max + 1, block_sums

@@ -440,7 +440,8 @@ __device__ __forceinline__ bool hash_has(const int32_t* H, int32_t m) {
 
 template <int kBits>
 __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_eu(kBits == kLightBitsBig ? 3 : 5))) void k_light(
-    const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems) {
+    const MatDev* __restrict__ mats, int nmats, const int64_t* __restrict__ item_base, int64_t nitems,
+    WorkState* __restrict__ ws) {
   constexpr int kHash = LightHash<kBits>::kHash;
   constexpr int kLightQ = LightHash<kBits>::kQ;
   __shared__ int32_t s_hash[kLightWaves][kHash];
@@ -456,7 +457,21 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
   // t in one direction is the reverse of a path t -> ... -> t+1 in the other, i.e. t+1 in U_t.
   __shared__ unsigned long long s_pub[kLightWaves];  // (item << 32) | (row + 1) of the last row published
   __shared__ uint8_t s_nxt[kLightWaves][8];          // by row mod 8: 1 in, 2 not in, 3 unknown
+  // Items. The 2,048-slot variant takes them in pairs (items 2p and 2p+1: the two directions of
+  // a block, or two blocks of a mirrored matrix) from a device counter, lowest rows first, so that
+  // waves that finish early take more (a fixed share per wave left the slowest wave 1.26x the mean
+  // on the 659,033-row stencil). The even wave of a sibling pair takes the pair and hands it to
+  // the odd wave: (fetch number << 32) | pair index; the odd wave acknowledges each fetch it has
+  // read, and the even wave takes the next pair only after that acknowledgement. The 1,024-slot
+  // variant strides over the items, item k to wave k mod waves (its shorter items measured faster
+  // that way: the handshake per item costs more than the imbalance saves).
+  __shared__ unsigned long long s_pair[kLightWaves / 2];
+  __shared__ uint32_t s_ack[kLightWaves / 2];
   if (threadIdx.x < kLightWaves) s_pub[threadIdx.x] = 0ull;
+  if (threadIdx.x < kLightWaves / 2) {
+    s_pair[threadIdx.x] = 0ull;
+    s_ack[threadIdx.x] = 0u;
+  }
   __syncthreads();
   const int sib = wave ^ 1;
 
@@ -465,6 +480,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
   uint64_t x_edges = 0, x_verts = 0, x_out = 0;  // wave-uniform
   int last_mat = -1;
 
+  constexpr bool kDynamic = kBits == kLightBitsBig;
+  const int64_t npairs = nitems / 2;  // nitems is even
+  const bool even = (wave & 1) == 0;
+  uint32_t fetch = 0;
   const int64_t gwave = (int64_t)blockIdx.x * kLightWaves + wave;
   const int64_t nwaves = (int64_t)gridDim.x * kLightWaves;
 
@@ -485,7 +504,66 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     x_edges = x_verts = x_out = 0;
   };
 
-  for (int64_t k = gwave; k < nitems; k += nwaves) {
+  for (int64_t round = 0;; ++round) {
+    int64_t k;
+    if constexpr (!kDynamic) {
+      k = gwave + round * nwaves;
+      if (k >= nitems) break;
+    } else {
+      int64_t pidx;
+      ++fetch;
+      // Both waits end once the other wave of the pair finishes the item it is on; they are bounded
+      // all the same (a wave that stops waiting ends and the run reports an error).
+      if (even) {
+        bool ok = true;
+        if (fetch > 1) {  // the odd wave has read the previous pair
+          ok = false;
+          for (int spin = 0; spin < (1 << 24); ++spin) {
+            if (__hip_atomic_load(&s_ack[wave >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == fetch - 1) {
+              ok = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          ok = __builtin_amdgcn_readfirstlane(ok ? 1 : 0) != 0;
+        }
+        if (!ok) {
+          if (lane == 0 && nmats > 0) atomicOr(&mats[0].acc[kAccScheduleError], 1ull);
+          break;
+        }
+        unsigned long long q = 0;
+        if (lane == 0) q = atomicAdd(&ws->light_next, 1ull);
+        q = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(q >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)q);
+        pidx = q < (unsigned long long)npairs ? (int64_t)q : npairs;
+        if (lane == 0) {  // always published, the last time with npairs (done)
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __hip_atomic_store(&s_pair[wave >> 1], ((unsigned long long)fetch << 32) | (uint32_t)pidx, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      } else {
+        unsigned long long v = 0;
+        bool got = false;
+        for (int spin = 0; spin < (1 << 24); ++spin) {
+          v = __hip_atomic_load(&s_pair[wave >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if ((uint32_t)(v >> 32) == fetch) {
+            got = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        got = __builtin_amdgcn_readfirstlane(got ? 1 : 0) != 0;
+        if (!got) {
+          if (lane == 0 && nmats > 0) atomicOr(&mats[0].acc[kAccScheduleError], 1ull);
+          break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        pidx = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        if (lane == 0) __hip_atomic_store(&s_ack[wave >> 1], fetch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (pidx >= npairs) break;
+      k = 2 * pidx + (even ? 0 : 1);
+    }
     const int mb = (nmats == 1) ? 0 : find_segment(item_base, nmats, k);
     if (mb != last_mat) {
       flush(last_mat);
@@ -2215,12 +2293,13 @@ hipError_t launch_transpose(int64_t nrows, int nmats, const int64_t* row_base, c
   return hipGetLastError();
 }
 
-hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, int grid, int bits,
-                        hipStream_t s) {
+hipError_t launch_light(const MatDev* mats, int nmats, const int64_t* item_base, int64_t nitems, WorkState* ws, int grid,
+                        int bits, hipStream_t s) {
   if (bits == kLightBitsBig)
-    hipLaunchKernelGGL(k_light<kLightBitsBig>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
+    hipLaunchKernelGGL(k_light<kLightBitsBig>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems,
+                       ws);
   else
-    hipLaunchKernelGGL(k_light<kLightBits>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems);
+    hipLaunchKernelGGL(k_light<kLightBits>, dim3(grid), dim3(kLightWaves * 64), 0, s, mats, nmats, item_base, nitems, ws);
   return hipGetLastError();
 }
 
