@@ -65,7 +65,8 @@ class Options(C.Structure):
                 ("device", C.c_int32), ("heavy_threads", C.c_int32), ("heavy_wgs_per_cu", C.c_int32),
                 ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32), ("chain_block", C.c_int32),
                 ("pair_directions", C.c_int32), ("autotune", C.c_int32), ("bitmap_in_hbm", C.c_int32),
-                ("split_blocks", C.c_int32), ("symmetric", C.c_int32), ("cu_share", C.c_int32)]
+                ("split_blocks", C.c_int32), ("symmetric", C.c_int32), ("cu_share", C.c_int32),
+                ("early_top", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -114,6 +115,7 @@ _SIGNATURES = {
     "gsofa_work_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),
     "gsofa_executed_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "gsofa_early_top": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "gsofa_plan_destroy": (None, [C.c_void_p]),
     "gsofa_read_matrix_market": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "gsofa_read_permutation": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
@@ -442,6 +444,13 @@ class Plan:
         _check(lib().gsofa_executed_work(self._h, w))
         return {"heavy_edges": w[0], "heavy_rows": w[1], "heavy_out": w[2],
                 "light_edges": w[3], "light_rows": w[4], "light_out": w[5]}
+
+    def early_top(self) -> dict:
+        """The last run's early launch of the top separator's items: item count (0: none), device
+        time (0 unless the run recorded stage events) and executed work."""
+        items, ms, w = C.c_int64(), C.c_double(), (C.c_uint64 * 3)()
+        _check(lib().gsofa_early_top(self._h, C.byref(items), C.byref(ms), w))
+        return {"items": items.value, "ms": ms.value, "edges": w[0], "rows": w[1], "out": w[2]}
 
     def structure(self):
         """(u_rowptr, u_col, l_colptr, l_row): strict U by rows, strict L by columns, sorted."""

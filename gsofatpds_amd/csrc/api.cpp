@@ -33,6 +33,7 @@ Options to_options(const gsofa_options* o) {
   r.split_blocks = o->split_blocks < 0 ? -1 : o->split_blocks;
   r.symmetric = o->symmetric < 0 ? -1 : 0;
   r.cu_share = o->cu_share > 0 ? o->cu_share : 0;
+  r.early_top = o->early_top < 0 ? -1 : (o->early_top > 0 ? 1 : 0);
   return r;
 }
 
@@ -431,6 +432,19 @@ int gsofa_work_split(gsofa_plan* p, uint64_t* heavy_edge_checks, uint64_t* heavy
   *heavy_edge_checks = p->results[0].heavy_w;
   *heavy_reach = p->results[0].heavy_r;
   *heavy_entries = p->results[0].heavy_nnz;
+  return GSOFA_OK;
+}
+
+int gsofa_early_top(gsofa_plan* p, int64_t* items, double* ms, uint64_t* work3) {
+  if (!p || !items || !ms || !work3) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
+  if (!p->have_result) {
+    int rc = gsofa_sync(p);
+    if (rc != GSOFA_OK) return rc;
+  }
+  *items = p->set.top_items();
+  int rc = p->set.top_ms(ms);
+  if (rc != GSOFA_OK) return rc;
+  for (int k = 0; k < 3; ++k) work3[k] = p->results[0].exec_top[k];
   return GSOFA_OK;
 }
 

@@ -119,6 +119,14 @@ int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std:
   return count;
 }
 
+int32_t top_separator_rows(const Prepared& P, const std::vector<int32_t>& cut) {
+  if (P.subtree.empty()) return -1;  // no cut was computed
+  int32_t best = -1;
+  for (int32_t c : cut)
+    if (c >= 0 && (best < 0 || P.subtree[c] > P.subtree[best])) best = c;
+  return best >= 0 && 2 * (int64_t)P.subtree[best] >= P.n ? best : -1;
+}
+
 // Paired items pay a bitmap save per row and win where chaining by "row s holds s-1" is rare
 // (unstructured patterns); structured ones (stencils in nested-dissection order) chain directly.
 bool pair_directions(const Prepared& P, int32_t mode) {
@@ -142,6 +150,10 @@ int device_info(int device, DeviceInfo& info) {
 }
 
 DeviceSet::~DeviceSet() {
+  if (side_) (void)hipStreamSynchronize(side_);
+  for (hipEvent_t e : {fork_, join_, top_ev_[0], top_ev_[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (side_) (void)hipStreamDestroy(side_);
   for (void* p : owned_) {
     if (alloc_stream_) (void)hipFreeAsync(p, alloc_stream_);
     else (void)hipFree(p);
@@ -319,6 +331,18 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
             "clear the run's state");
   if (q.prepare) GSOFA_TRY(enqueue_preparation(q, s));
   if (ev) GSOFA_SEQ(hipEventRecord(ev[1], s), "hipEventRecord");
+  if (q.n_top > 0) {
+    // the early launch: the top separators' items, beside the wave path
+    GSOFA_SEQ(hipEventRecord(q.fork, s), "hipEventRecord(fork)");
+    GSOFA_SEQ(hipStreamWaitEvent(q.side, q.fork, 0), "hipStreamWaitEvent(fork)");
+    GSOFA_SEQ(launch_top_setup(q.mats, q.nmats, q.ws_top, q.n_top, q.side), "early launch set-up");
+    if (q.top_ev) GSOFA_SEQ(hipEventRecord(q.top_ev[0], q.side), "hipEventRecord");
+    GSOFA_SEQ(launch_heavy(q.cfg.heavy_threads, q.top_grid, q.mats, q.ws_top, q.top_items, q.cfg.bw_max, q.cfg.qcap,
+                           q.n_max, q.paired_any, q.spill_top, nullptr, q.cfg.ucap, kAccTopExecEdges, q.side),
+              "k_heavy (early launch)");
+    if (q.top_ev) GSOFA_SEQ(hipEventRecord(q.top_ev[1], q.side), "hipEventRecord");
+    GSOFA_SEQ(hipEventRecord(q.join, q.side), "hipEventRecord(join)");
+  }
   GSOFA_SEQ(launch_light(q.mats, q.nmats, q.item_base, q.nitems, q.ws, q.cfg.light_grid, q.cfg.light_bits, s), "k_light");
   // cursor, block_sums and total are free again once the transpose is built
   GSOFA_SEQ(launch_heavy_order(q.mats, q.nblk_total, q.nmats, q.blkq_base, q.blk_flag, q.nb_max, q.bucket, q.cursor,
@@ -326,10 +350,11 @@ int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev) {
             "workgroup-path queue");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[2], s), "hipEventRecord");
   GSOFA_SEQ(launch_heavy(q.cfg.heavy_threads, q.cfg.heavy_grid, q.mats, q.ws, q.heavy_items, q.cfg.bw_max, q.cfg.qcap,
-                         q.n_max, q.paired_any, q.spill, q.hbm_bm, q.cfg.ucap, s),
+                         q.n_max, q.paired_any, q.spill, q.hbm_bm, q.cfg.ucap, kAccHeavyExecEdges, s),
             "k_heavy");
-  if (q.mirror_any) GSOFA_SEQ(launch_mirror(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_mirror");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[3], s), "hipEventRecord");
+  if (q.n_top > 0) GSOFA_SEQ(hipStreamWaitEvent(s, q.join, 0), "hipStreamWaitEvent(join)");
+  if (q.mirror_any) GSOFA_SEQ(launch_mirror(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_mirror");
   GSOFA_SEQ(launch_supernodes(q.mats, q.nmats, q.blk_base, q.nblocks_sup, s), "k_supernodes");
   if (ev) GSOFA_SEQ(hipEventRecord(ev[4], s), "hipEventRecord");
 #undef GSOFA_SEQ
@@ -416,10 +441,45 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
       ncut += c;
     }
   }
+  // The early launch: the top separator of large matrices (at least 1 / kCutShare of the set's
+  // rows) when the wave path uses its 2,048-slot variant, whose items are taken on demand (a few
+  // CUs held by the early launch then cost it little more than their share) and the bitmaps live
+  // in LDS. Automatic for counting-only runs; with the structure it measured even (the wave path
+  // beside it slows by about as much as the main launch gains), so only on request.
+  light_end_.assign(ns_.begin(), ns_.end());
+  std::vector<uint64_t> top_items;
+  const bool early = opt.early_top > 0 || (opt.early_top == 0 && !opt.structure);
+  if (early && cfg_.light_bits == kLightBitsBig && !hbm_bitmap_) {
+    std::vector<int32_t> cut;
+    for (int b = 0; b < nmats_; ++b) {
+      if (!has_cut[b] || (int64_t)ns_[b] * kCutShare < nrows_) continue;
+      cut.assign(cut_all.begin() + blkq_base_h_[b], cut_all.begin() + blkq_base_h_[b + 1]);
+      const int32_t top = top_separator_rows(*mats[b], cut);
+      if (top < 0) continue;
+      const int32_t K = opt_.chain_block;
+      // kTopItemsPerDir items per direction (a mirrored or paired matrix: one item for both),
+      // each over e consecutive blocks from the cut on: every item begins with a traversal of
+      // nearly the whole matrix and chains its other rows, so fewer, longer items execute fewer
+      // traversals; the launch runs beside the wave path and the main launch, not before them
+      const int ndir = (mirror_[b] || pair_directions(*mats[b], opt_.pair_directions)) ? 1 : 2;
+      const int64_t b0 = top / K, nblk = ((int64_t)ns_[b] + K - 1) / K - b0;
+      const int64_t e = std::min<int64_t>((nblk + kTopItemsPerDir - 1) / kTopItemsPerDir, kItemExtraMax + 1);
+      const int64_t nit = (nblk + e - 1) / e;
+      if ((int64_t)top_items.size() + ndir * nit > kTopMaxItems) continue;
+      for (int64_t k = 0; k < nit; ++k) {
+        const uint32_t row = k == 0 ? (uint32_t)top : (uint32_t)((b0 + k * e) * K);
+        for (int d = 0; d < ndir; ++d) top_items.push_back(encode_item((uint32_t)b, (uint32_t)d, row, 0u, (uint32_t)(e - 1)));
+      }
+      light_end_[b] = top;
+    }
+  }
+  n_top_ = (int64_t)top_items.size();
+  top_grid_ = (int)std::min<int64_t>(n_top_, heavy_grid_);
   for (int b = 0; b < nmats_; ++b) {
     // wave-path items: two per block (one per direction), or for a mirrored matrix one per block
-    // rounded up to even (the next matrix's items keep the parity the sibling exchange pairs by)
-    const int64_t nb = ((int64_t)ns_[b] + light_block_ - 1) / light_block_;
+    // rounded up to even (the next matrix's items keep the parity the sibling exchange pairs by);
+    // rows from light_end on are not the wave path's
+    const int64_t nb = ((int64_t)light_end_[b] + light_block_ - 1) / light_block_;
     item_base_h_[b + 1] = item_base_h_[b] + (mirror_[b] ? nb + (nb & 1) : 2 * nb);
   }
   nitems_ = item_base_h_[nmats_];
@@ -444,7 +504,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(item_base_d_, nmats_ + 1, "item base"));
   GSOFA_TRY(alloc(row_base_d_, nmats_ + 1, "row base"));
   GSOFA_TRY(alloc(blk_base_d_, nmats_ + 1, "block base"));
-  GSOFA_TRY(alloc(ws_, 1, "work state"));
+  GSOFA_TRY(alloc(ws_, n_top_ > 0 ? 2 : 1, "work state"));  // the second: the early launch's
   GSOFA_TRY(alloc(heavy_items_, 2 * (nblk_total_ + ncut), "escalation list"));
   if (ncut > 0) {
     GSOFA_TRY(alloc(cut_, (size_t)nblk_total_, "block cuts"));
@@ -453,8 +513,18 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // per resident workgroup: two frontier spill queues, the U list (beyond its LDS part) and the
   // saved bitmap of the last row (with paired items: a U list per direction and a ring of the
   // last saved rows' bitmaps per direction)
-  GSOFA_TRY(alloc(spill_, (size_t)heavy_grid_ * (size_t)heavy_region_ints(n_max_, paired_any_ != 0),
+  // (the early launch's workgroups have regions of their own after the main launch's)
+  GSOFA_TRY(alloc(spill_, (size_t)(heavy_grid_ + top_grid_) * (size_t)heavy_region_ints(n_max_, paired_any_ != 0),
                   "heavy spill regions"));
+  if (n_top_ > 0) {
+    GSOFA_TRY(alloc(top_items_, (size_t)n_top_, "early items"));
+    GSOFA_TRY(upload(top_items_, top_items.data(), top_items.size() * 8, "upload early items"));
+    GSOFA_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    GSOFA_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreateWithFlags");
+    GSOFA_HIP(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreateWithFlags");
+    GSOFA_HIP(hipEventCreate(&top_ev_[0]), "hipEventCreate");
+    GSOFA_HIP(hipEventCreate(&top_ev_[1]), "hipEventCreate");
+  }
   if (hbm_bitmap_) GSOFA_TRY(alloc(hbm_bm_, (size_t)heavy_grid_ * (size_t)bw_max_, "heavy bitmaps"));
   esc_words_ = (2 * nrows_ + 3) / 4;
   GSOFA_TRY(alloc(esc_, (size_t)esc_words_ * 4, "escalation flags"));
@@ -530,6 +600,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.sym = mirror_[b];
     M.paired = (!mirror_[b] && pair_directions(*mats[b], opt_.pair_directions)) ? 1 : 0;
     M.light_block = light_block_;
+    M.light_end = light_end_[b];
     M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall
                     : cfg_.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
   }
@@ -567,7 +638,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += nrows + 2 * nrows + 4 * 2 * nblk + 8 * 2 * nrows;  // flags, escalations, block flags, items
   b += 4 * nblk + 8 * 2 * nblk;                           // block cuts and their extra items
   b += 4 * (int64_t)nmax + 8 * ((int64_t)mats.size() + 1);  // block index histogram, chain block bases
-  b += 4 * heavy_grid * heavy_region_ints(nmax, paired);  // spill
+  b += 4 * (heavy_grid + kTopMaxItems) * heavy_region_ints(nmax, paired);  // spill (and the early launch's)
   if (place_bitmaps_in_hbm(nmax, opt, info)) b += 4 * (int64_t)cus * (((int64_t)nmax + 31) / 32);  // one per CU
   if (dup) b += 8 * nrows;
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
@@ -580,7 +651,18 @@ int DeviceSet::enqueue(hipStream_t s, hipEvent_t* ev) {
   GSOFA_HIP(hipSetDevice(opt_.device), "hipSetDevice");
   Sequence q = sequence();
   q.prepare = 0;  // built once by create()
+  q.top_ev = (ev && n_top_ > 0) ? top_ev_ : nullptr;
+  top_timed_ = q.top_ev != nullptr;
   return enqueue_sequence(q, s, ev);
+}
+
+int DeviceSet::top_ms(double* ms) {
+  *ms = 0.0;
+  if (!top_timed_) return GSOFA_OK;
+  float t = 0.f;
+  GSOFA_HIP(hipEventElapsedTime(&t, top_ev_[0], top_ev_[1]), "hipEventElapsedTime(early launch)");
+  *ms = t;
+  return GSOFA_OK;
 }
 
 Sequence DeviceSet::sequence() const {
@@ -620,6 +702,14 @@ Sequence DeviceSet::sequence() const {
   q.row_base = row_base_d_;
   q.blk_base = blk_base_d_;
   q.blkq_base = blkq_base_d_;
+  q.n_top = n_top_;
+  q.top_grid = top_grid_;
+  q.top_items = top_items_;
+  q.ws_top = n_top_ > 0 ? ws_ + 1 : nullptr;
+  q.spill_top = spill_ + (int64_t)heavy_grid_ * heavy_region_ints(n_max_, paired_any_ != 0);
+  q.side = side_;
+  q.fork = fork_;
+  q.join = join_;
   return q;
 }
 
@@ -642,6 +732,10 @@ int DeviceSet::read_results(std::vector<MatResult>& out) {
     r.heavy_r = a[kAccHeavyR];
     r.heavy_nnz = a[kAccHeavyNnz];
     for (int k = 0; k < 6; ++k) r.exec[k] = a[kAccHeavyExecEdges + k];
+    for (int k = 0; k < 3; ++k) {  // the workgroup path's work: both of its launches
+      r.exec_top[k] = a[kAccTopExecEdges + k];
+      r.exec[k] += r.exec_top[k];
+    }
     r.nsup = (int64_t)a[kAccNsup];
   }
   return GSOFA_OK;

@@ -37,6 +37,17 @@ constexpr int32_t kCutDivisor = 64;  // the automatic divisor
 // Automatic cuts apply to matrices holding at least 1 / kCutShare of the set's rows. Measured on
 // 256 matrices of 10,000-200,000 rows in one set: cutting every member 149 -> 160 ms per run.
 constexpr int32_t kCutShare = 4;
+// The start of a matrix's top separator among its block cuts (`cut`, from block_cuts): the cut row
+// with the largest subtree if that subtree holds at least half the rows, else -1. The items of the
+// rows from there on are the longest of the workgroup path (each begins with a traversal of
+// nearly the whole matrix); they need no other item's results and no lower row reads their
+// escalation flags, so with Options::early_top they run in a launch of their own that starts
+// beside the wave path.
+int32_t top_separator_rows(const Prepared& P, const std::vector<int32_t>& cut);
+// At most this many items go to the early launch (one workgroup each), and this many per
+// direction of a matrix (each over several chain blocks).
+constexpr int64_t kTopMaxItems = 32;
+constexpr int64_t kTopItemsPerDir = 3;
 
 // Whether the workgroup path runs both directions of a block together (Options::pair_directions).
 bool pair_directions(const Prepared& P, int32_t mode);
@@ -67,12 +78,16 @@ struct Options {
                                 // -1 never
   int32_t symmetric = 0;        // 0 traverse structurally symmetric patterns once (L mirrored from
                                 // U), -1 always both directions
+  int32_t early_top = 0;        // a large matrix's top-separator items start beside the wave path
+                                // (top_separator_rows): 0 in counting-only runs, 1 also with the
+                                // structure, -1 never
 };
 
 struct MatResult {
   uint64_t fill = 0, W = 0, R = 0, nnz_u = 0, nnz_l = 0, heavy = 0;
   uint64_t heavy_w = 0, heavy_r = 0, heavy_nnz = 0;  // the share of the workgroup path
   uint64_t exec[6] = {0, 0, 0, 0, 0, 0};            // executed edges/rows/outputs: heavy, light
+  uint64_t exec_top[3] = {0, 0, 0};                 // the early launch's share of exec[0..2]
   int64_t nsup = 0;
 };
 
@@ -120,11 +135,23 @@ struct Sequence {
   uint32_t *blk_flag = nullptr, *hbm_bm = nullptr;
   MatDev* mats = nullptr;
   int64_t *item_base = nullptr, *row_base = nullptr, *blk_base = nullptr, *blkq_base = nullptr;
+  // The early launch (n_top > 0): top_grid workgroups of k_heavy over the fixed items top_items
+  // on `side`, forked from the run's stream after its clears and joined before k_mirror and
+  // k_supernodes; its own work state and spill regions. top_ev (optional, 2 events) brackets it.
+  int64_t n_top = 0;
+  int top_grid = 0;
+  const uint64_t* top_items = nullptr;
+  WorkState* ws_top = nullptr;
+  int32_t* spill_top = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t* top_ev = nullptr;
 };
 // Enqueues the whole run on `s` (kernels only: no allocation, no memset node, no host
-// synchronisation). ev (optional, 5 events): before the run, after its clears (and the
-// preparation when q.prepare), after the wave path and the workgroup-path queue, after the
-// workgroup path, after supernodes.
+// synchronisation; with an early launch, also on q.side, joined back into `s`). ev (optional, 5
+// events): before the run, after its clears (and the preparation when q.prepare), after the wave
+// path and the workgroup-path queue, after the main workgroup-path launch, after the join,
+// k_mirror and supernodes.
 int enqueue_sequence(const Sequence& q, hipStream_t s, hipEvent_t* ev);
 // The input's device forms: ELL rows of A, and A^T with its ELL rows unless every matrix is
 // mirrored. Part of enqueue_sequence when q.prepare is set.
@@ -162,6 +189,9 @@ class DeviceSet {
   int qcap() const { return qcap_; }
   int heavy_wgs_per_cu() const { return heavy_wgs_per_cu_; }
   bool bitmap_in_hbm() const { return hbm_bitmap_; }
+  // Items of the early launch (0: none) and, after a run enqueued with stage events, its time.
+  int64_t top_items() const { return n_top_; }
+  int top_ms(double* ms);
 
  private:
   template <typename T>
@@ -218,6 +248,14 @@ class DeviceSet {
   std::vector<MatDev> mats_h_;
   std::vector<void*> owned_;
   hipStream_t alloc_stream_ = nullptr;
+  // the early launch of top-separator items (top_separator_rows)
+  int64_t n_top_ = 0;
+  int top_grid_ = 0;
+  uint64_t* top_items_ = nullptr;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr, top_ev_[2] = {nullptr, nullptr};
+  bool top_timed_ = false;  // the last run recorded top_ev_
+  std::vector<int32_t> light_end_;  // per matrix: MatDev::light_end
   std::vector<std::vector<int32_t>> staging_;  // host copies kept alive for async uploads
   int upload(void* dst, const void* src, size_t bytes, const char* what);
   int download(void* dst, const void* src, size_t bytes, const char* what);

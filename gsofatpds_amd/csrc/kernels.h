@@ -38,6 +38,11 @@ enum : int {
   kAccLightExecVerts = 17,  // rows expanded by k_light
   kAccLightExecOut = 18,    // F entries of the sources k_light finished
   kAccScheduleError = 19,   // nonzero: a wave-path wave stopped waiting for its sibling's next item
+  // the same three for the early launch of the top separator's items (kept apart so that each
+  // k_heavy launch is priced on its own work; kAccHeavyExec* counts the main launch only)
+  kAccTopExecEdges = 20,
+  kAccTopExecVerts = 21,
+  kAccTopExecOut = 22,
   kAccSlots = 24
 };
 
@@ -66,6 +71,9 @@ struct MatDev {
   int32_t sym;                    // 1: mirrored (structurally symmetric: direction 0 only, L's
                                   // counts and totals copied from U's by k_mirror)
   int32_t light_block;            // rows per wave-path item
+  int32_t light_end;              // the wave path runs rows [0, light_end); rows from light_end on
+                                  // (a top separator's, n: none) are fixed workgroup-path items of
+                                  // an early launch, escalated from the start
   int32_t light_limit;            // reached vertices before the wave path escalates a row
 };
 
@@ -77,12 +85,16 @@ struct WorkState {
   unsigned long long pad;
 };
 
-// Item encoding for the escalation list: rows before a cut (bit 63), matrix index (bits
-// 33..62), direction (bit 32), a row v (bits 0..31). The item runs rows v to the end of v's block
-// of chain_block rows (v is the block's first row, or a cut), or, with bit 63, the rows of v's
-// block before v.
-__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t row, uint32_t before_cut) {
-  return ((uint64_t)before_cut << 63) | ((uint64_t)mat << 33) | ((uint64_t)dir << 32) | (uint64_t)row;
+// Item encoding for the escalation list: rows before a cut (bit 63), extra blocks e (bits
+// 53..62), matrix index (bits 33..52: sets hold at most 2^20 matrices), direction (bit 32), a row
+// v (bits 0..31). The item runs rows v to the end of the e-th block after v's block of
+// chain_block rows (v is the block's first row, or a cut; e > 0 only for the early launch's
+// items), or, with bit 63, the rows of v's block before v.
+constexpr int kItemExtraMax = 1023;
+__host__ __device__ inline uint64_t encode_item(uint32_t mat, uint32_t dir, uint32_t row, uint32_t before_cut,
+                                                uint32_t extra = 0) {
+  return ((uint64_t)before_cut << 63) | ((uint64_t)extra << 53) | ((uint64_t)mat << 33) | ((uint64_t)dir << 32) |
+         (uint64_t)row;
 }
 
 }  // namespace gsofa
@@ -139,9 +151,14 @@ constexpr int64_t heavy_region_ints(int64_t n_max, bool paired_any) {
 }
 // hbm_bitmaps: null (bitmaps in LDS), or grid * bw_max words, one bitmap per workgroup
 // (1024-thread workgroups only).
+// exec_slot: the accumulator slot of the launch's first executed-work counter (kAccHeavyExecEdges,
+// or kAccTopExecEdges for the early launch).
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
-                        uint32_t* hbm_bitmaps, int32_t ucap, hipStream_t s);
+                        uint32_t* hbm_bitmaps, int32_t ucap, int32_t exec_slot, hipStream_t s);
+// Before the early launch: every matrix's rows [light_end, n) escalated (both directions), and
+// ws->heavy_count = count, ws->heavy_next = 0 (one block per matrix).
+hipError_t launch_top_setup(const MatDev* mats, int nmats, WorkState* ws, int64_t count, hipStream_t s);
 // Mirrored matrices: L counts = U counts, fill and workgroup-path entries doubled, nnz_L = nnz_U
 // (one thread per row, blocks as for launch_supernodes).
 hipError_t launch_mirror(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s);

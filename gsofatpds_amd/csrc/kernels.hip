@@ -579,7 +579,9 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     // Lowest rows first, so that their escalations are seen by the rows above (the escalation
     // rules below; measured 3 % faster than highest first on the 659,033-row stencil).
     const int32_t blk = sym ? (int32_t)local : (int32_t)(local >> 1);
-    const int32_t s0 = blk * lb, s1 = min(M.n, s0 + lb);
+    // rows from light_end on belong to the early launch, which runs with the 2,048-slot variant
+    // only (the 1,024-slot one keeps its registers: n)
+    const int32_t s0 = blk * lb, s1 = min(kBits == kLightBitsBig ? M.light_end : M.n, s0 + lb);
     // the U list is kept where it is read: for the structure, and for the window bits of L's
     // columns (direction 1, or a mirrored matrix's direction 0: L's column s is U's row s)
     const bool track_u = M.list_ptr[dir] != nullptr || dir == 1 || sym;
@@ -965,7 +967,9 @@ __global__ void k_heavy_hist(const MatDev* __restrict__ mats, int64_t nblk_total
     uint32_t dir;
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
-    const int parts = block_cut(mats[m], b) >= 0 ? 2 : 1;
+    // a block cut at light_end keeps only its part before the cut (the rest is an early item)
+    const int32_t cut = block_cut(mats[m], b);
+    const int parts = cut >= 0 && cut < mats[m].light_end ? 2 : 1;
     atomicAdd(&bucket[nb_max - 1 - b], parts);
     atomicAdd(&ws->heavy_count, (unsigned long long)parts);
   }
@@ -981,11 +985,15 @@ __global__ void k_heavy_scatter(const MatDev* __restrict__ mats, int64_t nblk_to
     int32_t b;
     if (!heavy_slot(mats, nblk_total, nmats, blkq_base, blk_flag, g, m, dir, b)) continue;
     const int32_t cut = block_cut(mats[m], b);
-    if (cut >= 0) {
+    if (cut >= 0 && cut < mats[m].light_end) {
       // a cut block: the part from the cut on (the larger subtree) goes first
       const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 2);
       heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)cut, 0u);
       heavy_items[pos + 1] = encode_item((uint32_t)m, dir, (uint32_t)cut, 1u);
+    } else if (cut >= 0) {
+      // cut at light_end: the part from the cut on runs in the early launch
+      const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
+      heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)cut, 1u);
     } else {
       const int32_t pos = atomicAdd(&cursor[nb_max - 1 - b], 1);
       heavy_items[pos] = encode_item((uint32_t)m, dir, (uint32_t)(b * mats[m].chain_block), 0u);
@@ -1029,7 +1037,7 @@ struct Totals {  // cleared with Totals{}
 };
 // Every k_heavy total is also the workgroup path's own share: W, R and F entries found there.
 
-__device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals& t) {
+__device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals& t, int32_t exec_slot) {
   if (mb < 0) return;
   unsigned long long* acc = mats[mb].acc;
   if (t.fill) atomicAdd(&acc[kAccFill], t.fill);
@@ -1041,9 +1049,9 @@ __device__ __forceinline__ void flush_totals(const MatDev* mats, int mb, Totals&
   if (t.w) atomicAdd(&acc[kAccHeavyW], t.w);
   if (t.r) atomicAdd(&acc[kAccHeavyR], t.r);
   if (t.nnz_u + t.nnz_l) atomicAdd(&acc[kAccHeavyNnz], t.nnz_u + t.nnz_l);
-  if (t.x_edges) atomicAdd(&acc[kAccHeavyExecEdges], t.x_edges);
-  if (t.x_verts) atomicAdd(&acc[kAccHeavyExecVerts], t.x_verts);
-  if (t.x_out) atomicAdd(&acc[kAccHeavyExecOut], t.x_out);
+  if (t.x_edges) atomicAdd(&acc[exec_slot], t.x_edges);
+  if (t.x_verts) atomicAdd(&acc[exec_slot + 1], t.x_verts);
+  if (t.x_out) atomicAdd(&acc[exec_slot + 2], t.x_out);
   t = Totals{};
 }
 
@@ -1116,7 +1124,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                     const uint64_t* __restrict__ heavy_items,
                                                     int32_t bw_max, int32_t qcap, int64_t n_max,
                                                     int32_t* __restrict__ spill_all,
-                                                    uint32_t* __restrict__ hbm_bitmaps, int32_t ucap) {
+                                                    uint32_t* __restrict__ hbm_bitmaps, int32_t ucap,
+                                                    int32_t exec_slot) {
   static_assert(kEll == 16, "the window scan maps one row to 16 lanes");
   extern __shared__ uint32_t dyn[];
   constexpr int kPfRounds = ((kHeavyWin + 1) * kEll + kThreads - 1) / kThreads;
@@ -1197,7 +1206,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const unsigned long long idx = s_idx;
     if (idx >= total) break;
     const uint64_t item = heavy_items[idx];
-    const int mb = (int)((item >> 33) & 0x3fffffffu);
+    const int mb = (int)((item >> 33) & 0xfffffu);
     const int item_dir = (int)((item >> 32) & 1);
     const int32_t v = (int32_t)(item & 0xffffffffu);
     const int before_cut = (int)(item >> 63);
@@ -1210,7 +1219,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     // v (decoded from the item alone: a table read here costs the traversal registers)
     const int32_t vb = v - v % M.chain_block;
     const int32_t r0 = before_cut ? vb : v;
-    const int32_t r1 = before_cut ? v : min(n, vb + M.chain_block);
+    const int32_t r1 = before_cut ? v : (int32_t)min((int64_t)n, vb + (int64_t)M.chain_block * (1 + (int32_t)((item >> 53) & 0x3ffu)));
     const bool want_list = M.list_ptr[0] != nullptr;  // both directions or neither
 
     // The leader's state of this item, per direction (index = direction; a plain item uses its
@@ -1785,7 +1794,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 s_facts[d][kSavedRing] = f;
                 if (save) s_facts[d][wslot] = f;
                 if (mb != last_mat) {
-                  flush_totals(mats, last_mat, tot);
+                  flush_totals(mats, last_mat, tot, exec_slot);
                   last_mat = mb;
                 }
                 gstore(M.cnt[d] + src, (int32_t)U);
@@ -2164,7 +2173,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       }
     }
   }
-  if (tid == 0) flush_totals(mats, last_mat, tot);
+  if (tid == 0) flush_totals(mats, last_mat, tot, exec_slot);
 }
 #undef GSOFA_SLOT
 #undef GSOFA_SV
@@ -2236,6 +2245,22 @@ __global__ __launch_bounds__(256) void k_mirror(const MatDev* __restrict__ mats,
   }
 }
 
+// Before the early launch of the top separators' items: their rows escalated in both directions
+// (k_heavy runs a window's escalated rows), and the launch's own work counter. One block per
+// matrix; matrices without such rows (light_end == n) do nothing.
+__global__ __launch_bounds__(256) void k_top_setup(const MatDev* __restrict__ mats, WorkState* __restrict__ ws,
+                                                   int64_t count) {
+  const MatDev& M = mats[blockIdx.x];
+  for (int32_t r = M.light_end + (int32_t)threadIdx.x; r < M.n; r += (int32_t)blockDim.x) {
+    M.esc[0][r] = 1;
+    M.esc[1][r] = 1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws->heavy_count = (unsigned long long)count;
+    ws->heavy_next = 0ull;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launchers (the engine calls these; kernels stay private to this file)
 // ------------------------------------------------------------------------------------------
@@ -2249,6 +2274,11 @@ static int grid_for(int64_t work, int block, int cap) {
 hipError_t launch_mirror(const MatDev* mats, int nmats, const int64_t* blk_base, int64_t nblocks, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_mirror, dim3((unsigned)nblocks), dim3(256), 0, s, mats, nmats, blk_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_top_setup(const MatDev* mats, int nmats, WorkState* ws, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_top_setup, dim3((unsigned)nmats), dim3(256), 0, s, mats, ws, count);
   return hipGetLastError();
 }
 
@@ -2367,11 +2397,11 @@ hipError_t heavy_set_lds_limit(int threads, size_t bytes, bool hbm_bitmap) {
 
 hipError_t launch_heavy(int threads, int grid, const MatDev* mats, WorkState* ws, const uint64_t* heavy_items,
                         int32_t bw_max, int32_t qcap, int64_t n_max, int32_t paired_any, int32_t* spill,
-                        uint32_t* hbm_bitmaps, int32_t ucap, hipStream_t s) {
+                        uint32_t* hbm_bitmaps, int32_t ucap, int32_t exec_slot, hipStream_t s) {
   const bool hbm = hbm_bitmaps != nullptr;
   const void* f = heavy_fn(threads, paired_any != 0, hbm);
   if (!f) return hipErrorInvalidValue;
-  void* args[] = {&mats, &ws, &heavy_items, &bw_max, &qcap, &n_max, &spill, &hbm_bitmaps, &ucap};
+  void* args[] = {&mats, &ws, &heavy_items, &bw_max, &qcap, &n_max, &spill, &hbm_bitmaps, &ucap, &exec_slot};
   return hipLaunchKernel(f, dim3(grid), dim3(threads), args,
                          heavy_lds_bytes(bw_max, qcap, hbm, ucap, paired_any != 0), s);
 }

@@ -66,7 +66,8 @@ bool same_options(const Options& a, const Options& b) {
          a.device == b.device && a.heavy_threads == b.heavy_threads && a.heavy_wgs_per_cu == b.heavy_wgs_per_cu &&
          a.light_wgs_per_cu == b.light_wgs_per_cu && a.qcap == b.qcap && a.chain_block == b.chain_block &&
          a.pair_directions == b.pair_directions && a.bitmap_in_hbm == b.bitmap_in_hbm && a.cu_share == b.cu_share &&
-         a.split_blocks == b.split_blocks && a.symmetric == b.symmetric;
+         a.split_blocks == b.split_blocks && a.symmetric == b.symmetric &&
+         a.early_top == b.early_top;
 }
 
 // Host bytes a request keeps alive (its prepared pattern).

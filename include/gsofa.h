@@ -168,6 +168,15 @@ typedef struct gsofa_options {
   int32_t cu_share;         /* 0 (default): persistent grids sized for every CU; k > 0: for k
                                CUs, so that work the caller runs beside it on other CUs (a
                                stream with a CU mask) keeps its share of the device            */
+  int32_t early_top;        /* the rows from the start of a large matrix's top separator (a
+                               block cut whose subtree holds half the rows; matrices of at least
+                               a quarter of the set's rows, over 262,144 rows, bitmaps in LDS)
+                               run as a few fixed workgroup-path items, three per direction over
+                               consecutive blocks, in a launch of their own on an internal
+                               stream, which starts with the wave path and joins the run before
+                               supernode detection. 0 (default): in counting-only runs (structure
+                               0); 1: also with the structure; -1 never. Results do not depend
+                               on it                                                           */
 } gsofa_options;
 
 typedef struct gsofa_result {
@@ -229,9 +238,15 @@ int gsofa_supernodal_from_structure(int32_t n, const uint8_t* flags, const int64
                                     int32_t* usub);
 /* Device time of the last run's stages, from events on the run's stream: ms[0] the run's clears
  * (the pattern's device forms, ELL rows and the transpose, are built once when the plan or batch
- * is created), ms[1] the wave (hash) path, ms[2] the workgroup (bitmap) path, ms[3] supernode
- * detection. No counterpart in the reference (its only timers are wtime()). */
+ * is created), ms[1] the wave (hash) path and the workgroup-path queue, ms[2] the workgroup
+ * (bitmap) path's main launch, ms[3] the wait for the early launch (if it is still running),
+ * supernode detection. No counterpart in the reference (its only timers are wtime()). */
 int gsofa_stage_times(gsofa_plan* plan, double* ms);
+/* The early launch of the top separator's items (gsofa_options.early_top) in the last run: its
+ * item count (0: none), its device time (events on its own stream, which starts with the wave
+ * path) and its executed work3 = {neighbour inspections, rows expanded, F entries written}
+ * (included in gsofa_executed_work's workgroup-path figures). No counterpart in the reference. */
+int gsofa_early_top(gsofa_plan* plan, int64_t* items, double* ms, uint64_t* work3);
 /* The workgroup path's share of the last run: row-traversal edge checks, |R_i| sum, and strict
  * F entries (both directions). Used to price that kernel alone. */
 int gsofa_work_split(gsofa_plan* plan, uint64_t* heavy_edge_checks, uint64_t* heavy_reach, uint64_t* heavy_entries);

@@ -612,3 +612,77 @@ def test_full_size_structure_properties():
         plan2.run()
         up2, ui2, lp2, li2 = plan2.structure()
     assert np.array_equal(up, up2) and np.array_equal(ui, ui2) and np.array_equal(lp, lp2) and np.array_equal(li, li2)
+
+
+def _early_case(case):
+    if case == "grid2d_600_unsym":  # both directions, one item per direction and block
+        return g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=600, ny=600, order=g.ORDER_ND,
+                          drop_side=g.DROP_LOWER, drop_ppm=100000, seed=12), {}
+    if case == "grid2d_600_mirrored":  # direction 0 only, L mirrored from U
+        return g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=600, ny=600, order=g.ORDER_ND, seed=13), {}
+    # paired items: one item runs both directions of a block
+    return (g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=600, ny=600, order=g.ORDER_ND,
+                       drop_side=g.DROP_UPPER, drop_ppm=50000, seed=14), {"pair_directions": 1})
+
+
+@pytest.mark.parametrize("case", ["grid2d_600_unsym", "grid2d_600_mirrored", "grid2d_600_paired"])
+def test_early_top_launch_changes_nothing(case):
+    """The top separator's items run in an early launch on a stream of their own, beside the wave
+    path; the results (totals, structure, counts, supernode flags) are those of a run without it,
+    over several runs of one plan."""
+    p, opt = _early_case(case)
+    say(f"[early top] {case} n={p.n}")
+    out = []
+    for early in (1, -1):  # forced on with the structure (automatic only when counting)
+        with g.Plan(p, structure=1, early_top=early, **opt) as plan:
+            for k in range(3 if early == 1 else 1):
+                plan.run()
+                r = plan.result()
+                up, ui, lp, li = plan.structure()
+                out.append(((r.fill_count, r.nnz_F, r.nnz_L, r.nnz_U, r.edge_checks, r.reach, r.supernodes_counted),
+                            oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
+                            g.fnv1a_array(plan.supernode_flags())))
+                st, et = plan.stage_times(), plan.early_top()
+                say(f"[early top] {case} early={early} run {k}: {r.time_ms:.2f} ms, {st}, {et}")
+                if early == 1:
+                    assert et["items"] > 0 and et["ms"] > 0 and et["edges"] > 0, et
+                else:
+                    assert et["items"] == 0 and et["edges"] == 0, et
+    assert all(o == out[-1] for o in out), [o[0] for o in out]
+    # counting only: automatic, and the counts, flags and totals of the runs above
+    with g.Plan(p, structure=0, **opt) as plan:
+        plan.run()
+        r = plan.result()
+        assert plan.early_top()["items"] > 0
+        assert ((r.fill_count, r.nnz_F, r.nnz_L, r.nnz_U, r.edge_checks, r.reach, r.supernodes_counted),
+                g.fnv1a_array(plan.nzu()), g.fnv1a_array(plan.supernode_flags())) == \
+            (out[-1][0], out[-1][2], out[-1][3])
+
+
+def test_early_top_launch_in_a_batch_graph():
+    """A batch whose largest member holds the set's majority takes the early launch inside its
+    captured graph (a fork to the internal stream and a join); every replay equals plain plans
+    without it."""
+    big, _ = _early_case("grid2d_600_unsym")
+    pats = [big] + [g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=60 + 7 * k, ny=50 + 5 * k,
+                               order=g.ORDER_ND, seed=k) for k in range(4)]
+    keys = ("fill_count", "nnz_F", "nnz_L", "nnz_U", "edge_checks", "reach", "supernodes")
+    want = []
+    for p in pats:
+        with g.Plan(p, early_top=-1) as plan:
+            plan.run()
+            r = plan.result()
+            want.append(tuple(getattr(r, k) for k in keys) + (g.fnv1a_array(plan.nzu()),
+                                                              g.fnv1a_array(plan.supernode_flags())))
+    b = g.Batch(pats)  # counting only: the early launch is automatic
+    try:
+        for rep in range(3):
+            say(f"[early top batch] replay {rep}")
+            b.run().sync()
+            got = [tuple(getattr(b.result(i), k) for k in keys) + (g.fnv1a_array(b.nzu(i)),
+                                                                   g.fnv1a_array(b.supernode_flags(i)))
+                   for i in range(len(pats))]
+            assert got == want
+            g.device_synchronize()
+    finally:
+        b.close()
