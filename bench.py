@@ -313,13 +313,16 @@ def main():
         plan.run().sync()
         log(f"[bench] warmup {i}: {plan.result().time_ms:.3f} ms")
     g.device_synchronize()
-    # the timed steps: a run and its completion with the totals read back, nothing else
+    # the timed steps: a run and its completion with the totals read back, nothing else (no
+    # stage events between the kernels)
+    plan.stage_timing(False)
     t0 = time.perf_counter()
     for i in range(args.steps):
         plan.run().sync()
     g.device_synchronize()
     wall = time.perf_counter() - t0
     r = plan.result()
+    plan.stage_timing(True)
     # the per-stage breakdown (HIP events on the run's stream) from as many further runs
     stage_acc = {"prep_ms": 0.0, "light_ms": 0.0, "heavy_ms": 0.0, "supernodes_ms": 0.0}
     event_ms = early_ms = 0.0

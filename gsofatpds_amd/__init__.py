@@ -116,6 +116,7 @@ _SIGNATURES = {
                                    C.POINTER(C.c_uint64)]),
     "gsofa_executed_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "gsofa_early_top": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "gsofa_plan_stage_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "gsofa_plan_destroy": (None, [C.c_void_p]),
     "gsofa_read_matrix_market": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "gsofa_read_permutation": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
@@ -422,6 +423,12 @@ class Plan:
         out = np.zeros(self.pattern.n, np.uint8)
         _check(lib().gsofa_supernodes_get(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+    def stage_timing(self, on: bool) -> "Plan":
+        """Record the stage events in later runs (the default) or not (a few microseconds of device
+        time per event; stage_times() then raises)."""
+        _check(lib().gsofa_plan_stage_timing(self._h, 1 if on else 0))
+        return self
 
     def stage_times(self) -> dict:
         ms = (C.c_double * 4)()

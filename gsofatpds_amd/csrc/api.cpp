@@ -176,6 +176,8 @@ struct gsofa_plan {
   std::vector<MatResult> results;
   bool have_result = false;
   bool ran = false;  // a run was enqueued: results and timings exist only after one
+  bool stage_timing = true;  // record the stage events in gsofa_run
+  bool staged = false;       // the last run recorded them
   // host copy of the last run's structure and flags for the supernodal export (until the next run)
   bool sn_cached = false;
   std::vector<int64_t> sn_up, sn_lp;
@@ -323,8 +325,9 @@ int gsofa_run(gsofa_plan* p, void* hip_stream) {
     if (rc != GSOFA_OK) return rc;
   }
   if (hipEventRecord(p->timing.start, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
-  int rc = p->set.enqueue(s, p->stage);
+  int rc = p->set.enqueue(s, p->stage_timing ? p->stage : nullptr);
   if (rc != GSOFA_OK) return rc;
+  p->staged = p->stage_timing;
   if (hipEventRecord(p->timing.stop, s) != hipSuccess) return fail(GSOFA_ERR_HIP, "hipEventRecord");
   p->timing.stream = s;
   p->timing.pending = true;
@@ -408,12 +411,19 @@ int gsofa_structure_get(gsofa_plan* p, int64_t* u_rowptr, int32_t* u_col, int64_
   return GSOFA_OK;
 }
 
+int gsofa_plan_stage_timing(gsofa_plan* p, int32_t on) {
+  if (!p) return fail(GSOFA_ERR_INVALID_ARG, "plan is null");
+  p->stage_timing = on != 0;
+  return GSOFA_OK;
+}
+
 int gsofa_stage_times(gsofa_plan* p, double* ms) {
   if (!p || !ms) return fail(GSOFA_ERR_INVALID_ARG, "null argument");
   if (!p->have_result) {
     int rc = gsofa_sync(p);
     if (rc != GSOFA_OK) return rc;
   }
+  if (!p->staged) return fail(GSOFA_ERR_STATE, "the last run recorded no stage times (gsofa_plan_stage_timing)");
   for (int k = 0; k < 4; ++k) {
     float t = 0.f;
     if (hipEventElapsedTime(&t, p->stage[k], p->stage[k + 1]) != hipSuccess)

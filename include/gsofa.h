@@ -242,6 +242,10 @@ int gsofa_supernodal_from_structure(int32_t n, const uint8_t* flags, const int64
  * (bitmap) path's main launch, ms[3] the wait for the early launch (if it is still running),
  * supernode detection. No counterpart in the reference (its only timers are wtime()). */
 int gsofa_stage_times(gsofa_plan* plan, double* ms);
+/* Whether gsofa_run records the stage events above (on = 1, the default) or not (0: each event
+ * between kernels costs a few microseconds of device time; gsofa_stage_times then reports
+ * GSOFA_ERR_STATE and gsofa_early_top a time of 0). No counterpart in the reference. */
+int gsofa_plan_stage_timing(gsofa_plan* plan, int32_t on);
 /* The early launch of the top separator's items (gsofa_options.early_top) in the last run: its
  * item count (0: none), its device time (events on its own stream, which starts with the wave
  * path) and its executed work3 = {neighbour inspections, rows expanded, F entries written}

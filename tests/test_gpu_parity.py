@@ -402,6 +402,23 @@ def test_results_before_any_run_are_a_state_error():
         b.close()
 
 
+def test_stage_timing_switch():
+    """Runs without stage events give the same results; their stage times are a state error
+    until a run records them again."""
+    p = g.generate_workload(g.WL_GRID2D_4096)
+    with g.Plan(p, structure=1) as plan:
+        plan.run()
+        want = (plan.result().fill_count, g.fnv1a_array(plan.nzu()), g.fnv1a_array(plan.supernode_flags()))
+        assert plan.stage_times()["heavy_ms"] > 0
+        plan.stage_timing(False).run()
+        assert (plan.result().fill_count, g.fnv1a_array(plan.nzu()), g.fnv1a_array(plan.supernode_flags())) == want
+        with pytest.raises(g.GsofaError) as e:
+            plan.stage_times()
+        assert e.value.code == g.ERR_STATE
+        plan.stage_timing(True).run()
+        assert plan.result().fill_count == want[0] and plan.stage_times()["heavy_ms"] > 0
+
+
 def _large(name):
     import hashlib
     import json
