@@ -161,6 +161,9 @@ int enqueue_preparation(const Sequence& q, hipStream_t s);
 // each kernel.
 class DeviceSet {
  public:
+  DeviceSet() = default;
+  DeviceSet(const DeviceSet&) = delete;  // owns device memory, a stream and events
+  DeviceSet& operator=(const DeviceSet&) = delete;
   ~DeviceSet();
   // alloc_stream != null: stream-ordered allocations and uploads on that stream (no device-wide
   // synchronisation; used by the server), freed on the same stream by the destructor.
