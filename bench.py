@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--workload", default="grid2d_659033", choices=["grid2d_659033", "grid3d_110592", "grid2d_4096"])
     ap.add_argument("--chunk", type=int, default=128)
     ap.add_argument("--no-structure", action="store_true")
+    ap.add_argument("--early-top", action="store_true",
+                    help="start the top separator's items beside the wave path (gsofa_options.early_top = 1)")
     ap.add_argument("--cpu-sample", type=int, default=150000,
                     help="sources in the CPU baseline's seeded sample (each of 3 timed passes)")
     ap.add_argument("--cpu-full", action="store_true", help="CPU baseline over all sources (minutes)")
@@ -307,7 +309,7 @@ def main():
     info = g.device_query(0)
     log(f"[bench] device {info}")
     structure = not args.no_structure
-    plan = g.Plan(p, chunk_size=args.chunk, structure=int(structure))
+    plan = g.Plan(p, chunk_size=args.chunk, structure=int(structure), early_top=1 if args.early_top else 0)
     log("[bench] plan created")
     for i in range(args.warmup):
         plan.run().sync()

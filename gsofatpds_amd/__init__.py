@@ -66,7 +66,7 @@ class Options(C.Structure):
                 ("light_wgs_per_cu", C.c_int32), ("queue_capacity", C.c_int32), ("chain_block", C.c_int32),
                 ("pair_directions", C.c_int32), ("autotune", C.c_int32), ("bitmap_in_hbm", C.c_int32),
                 ("split_blocks", C.c_int32), ("symmetric", C.c_int32), ("cu_share", C.c_int32),
-                ("early_top", C.c_int32)]
+                ("early_top", C.c_int32), ("pre_escalate", C.c_int32)]
 
 
 class Result(C.Structure):

@@ -34,6 +34,7 @@ Options to_options(const gsofa_options* o) {
   r.symmetric = o->symmetric < 0 ? -1 : 0;
   r.cu_share = o->cu_share > 0 ? o->cu_share : 0;
   r.early_top = o->early_top < 0 ? -1 : (o->early_top > 0 ? 1 : 0);
+  r.pre_escalate = o->pre_escalate < 0 ? -1 : 0;
   return r;
 }
 

@@ -119,6 +119,10 @@ int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std:
   return count;
 }
 
+int32_t light_limit_for(int32_t n, int light_bits) {
+  return n <= kLightSmallRows ? kLightLimitSmall : light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
+}
+
 int32_t top_separator_rows(const Prepared& P, const std::vector<int32_t>& cut) {
   if (P.subtree.empty()) return -1;  // no cut was computed
   int32_t best = -1;
@@ -441,15 +445,15 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
       ncut += c;
     }
   }
-  // The early launch: the top separator of large matrices (at least 1 / kCutShare of the set's
-  // rows) when the wave path uses its 2,048-slot variant, whose items are taken on demand (a few
-  // CUs held by the early launch then cost it little more than their share) and the bitmaps live
-  // in LDS. Automatic for counting-only runs; with the structure it measured even (the wave path
-  // beside it slows by about as much as the main launch gains), so only on request.
+  // The early launch (on request): the top separator of large matrices (at least 1 / kCutShare
+  // of the set's rows) when the wave path uses its 2,048-slot variant, whose items are taken on
+  // demand (a few CUs held by the early launch then cost it little more than their share) and
+  // the bitmaps live in LDS. Nothing orders its dispatch before the wave path's: when the wave
+  // path fills the CUs first, the early launch runs after it, beside the main launch, and the
+  // run takes longer than without it.
   light_end_.assign(ns_.begin(), ns_.end());
   std::vector<uint64_t> top_items;
-  const bool early = opt.early_top > 0 || (opt.early_top == 0 && !opt.structure);
-  if (early && cfg_.light_bits == kLightBitsBig && !hbm_bitmap_) {
+  if (opt.early_top > 0 && cfg_.light_bits == kLightBitsBig && !hbm_bitmap_) {
     std::vector<int32_t> cut;
     for (int b = 0; b < nmats_; ++b) {
       if (!has_cut[b] || (int64_t)ns_[b] * kCutShare < nrows_) continue;
@@ -531,6 +535,36 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   GSOFA_TRY(alloc(blk_flag_, (size_t)(2 * nblk_total_), "block flags"));
   GSOFA_TRY(alloc(blkq_base_d_, nmats_ + 1, "chain block base"));
   GSOFA_TRY(alloc(bucket_, nb_max_, "block index histogram"));
+  // Pre-escalation: a row whose elimination subtree of A + A^T holds more vertices than the
+  // escalation limit goes to the workgroup path without a wave-path attempt. A mirrored matrix's
+  // reached set holds the whole subtree, so the rule is exact there; a directed reach can be
+  // smaller, but on the unsymmetric stencils it sent 11 of 98,731 (659,033 rows) and 53 of
+  // 76,125 (110,592 rows) workgroup-path traversals there needlessly. Paired matrices
+  // (unstructured: reaches far smaller than their subtrees) keep every attempt. Only matrices
+  // with more rows than the limit carry the subtree sizes.
+  std::vector<int32_t> pre_size((size_t)nmats_, INT32_MAX);
+  bool any_pre = false;
+  if (opt.pre_escalate != -1) {
+    for (int b = 0; b < nmats_; ++b) {
+      if (!mirror_[b] && pair_directions(*mats[b], opt_.pair_directions)) continue;
+      const int32_t limit = light_limit_for(ns_[b], cfg_.light_bits);
+      if (ns_[b] <= limit) continue;
+      pre_size[b] = limit;
+      any_pre = true;
+    }
+  }
+  if (any_pre) {
+    GSOFA_TRY(alloc(subtree_d_, (size_t)nrows_, "subtree sizes"));
+    std::vector<int32_t> sub((size_t)nrows_, 0);
+    for (int b = 0; b < nmats_; ++b) {
+      if (pre_size[b] == INT32_MAX) continue;
+      const Prepared& P = *mats[b];
+      if (P.subtree.empty()) subtree_sizes(P, P.subtree);
+      std::copy(P.subtree.begin(), P.subtree.end(), sub.begin() + row_base_h_[b]);
+    }
+    GSOFA_TRY(upload(subtree_d_, sub.data(), sub.size() * 4, "upload subtree sizes"));
+    if (alloc_stream_) GSOFA_HIP(hipStreamSynchronize(alloc_stream_), "hipStreamSynchronize(subtree sizes)");
+  }
   if (any_dup) {
     GSOFA_TRY(alloc(dup_u_, nrows_, "duplicate surplus (upper)"));
     GSOFA_TRY(alloc(dup_all_, nrows_, "duplicate surplus (row)"));
@@ -601,8 +635,9 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
     M.paired = (!mirror_[b] && pair_directions(*mats[b], opt_.pair_directions)) ? 1 : 0;
     M.light_block = light_block_;
     M.light_end = light_end_[b];
-    M.light_limit = ns_[b] <= kLightSmallRows ? kLightLimitSmall
-                    : cfg_.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
+    M.light_limit = light_limit_for(ns_[b], cfg_.light_bits);
+    M.subtree = pre_size[b] != INT32_MAX ? subtree_d_ + rb : nullptr;
+    M.pre_size = pre_size[b];
   }
   GSOFA_TRY(upload(mats_d_, mats_h_.data(), sizeof(MatDev) * nmats_, "upload table"));
   // The pattern is fixed for the set's lifetime, and so are its device forms (ELL rows, A^T): built

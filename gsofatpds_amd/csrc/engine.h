@@ -44,6 +44,8 @@ constexpr int32_t kCutShare = 4;
 // escalation flags, so with Options::early_top they run in a launch of their own that starts
 // beside the wave path.
 int32_t top_separator_rows(const Prepared& P, const std::vector<int32_t>& cut);
+// The wave path's escalation limit for a matrix of n rows under a hash-set variant.
+int32_t light_limit_for(int32_t n, int light_bits);
 // At most this many items go to the early launch (one workgroup each), and this many per
 // direction of a matrix (each over several chain blocks).
 constexpr int64_t kTopMaxItems = 32;
@@ -78,9 +80,10 @@ struct Options {
                                 // -1 never
   int32_t symmetric = 0;        // 0 traverse structurally symmetric patterns once (L mirrored from
                                 // U), -1 always both directions
-  int32_t early_top = 0;        // a large matrix's top-separator items start beside the wave path
-                                // (top_separator_rows): 0 in counting-only runs, 1 also with the
-                                // structure, -1 never
+  int32_t pre_escalate = 0;     // 0: rows of unpaired matrices whose elimination subtree exceeds
+                                // the escalation limit skip the wave path; -1 never
+  int32_t early_top = 0;        // 1: a large matrix's top-separator items start beside the wave
+                                // path (top_separator_rows); 0 or -1: not
 };
 
 struct MatResult {
@@ -255,6 +258,7 @@ class DeviceSet {
   int64_t n_top_ = 0;
   int top_grid_ = 0;
   uint64_t* top_items_ = nullptr;
+  int32_t* subtree_d_ = nullptr;  // nrows: subtree sizes for pre-escalation (when some matrix uses it)
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ = nullptr, join_ = nullptr, top_ev_[2] = {nullptr, nullptr};
   bool top_timed_ = false;  // the last run recorded top_ev_

@@ -75,6 +75,9 @@ struct MatDev {
                                   // (a top separator's, n: none) are fixed workgroup-path items of
                                   // an early launch, escalated from the start
   int32_t light_limit;            // reached vertices before the wave path escalates a row
+  const int32_t* subtree;         // n: elimination-subtree sizes of A + A^T (null: none)
+  int32_t pre_size;               // a row whose subtree exceeds this escalates without a
+                                  // wave-path attempt (its reached set would exceed the limit)
 };
 
 // Scheduler state of one launch set (plain matrix or batch).

@@ -599,6 +599,9 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
     int self_st = 3;           // mirrored: is src in X_{src-1}? 1 in, 2 not in, 3 unknown
     for (int32_t src = s0; src < s1; ++src) {
       const int32_t chunk_end = min(M.n, (src / M.chunk + 1) * M.chunk);
+      // A row whose elimination subtree exceeds pre_size escalates at once: its reached set holds
+      // its whole subtree (mirrored matrices), or nearly (pre_size leaves a margin otherwise).
+      const bool pre = M.subtree != nullptr && gload(M.subtree + src) > M.pre_size;
       RowCursor rc;
       // the source row, one ELL slot per lane (lanes 0..15); slot 15 <= -2 encodes a CSR tail
       const int32_t sm = lane < kEll ? gload(M.ell[dir] + (int64_t)src * kEll + lane) : kPad;
@@ -620,7 +623,7 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       bool chained = __ballot(has_prev_in_row) != 0;
       // otherwise (when X_{src-1} is at hand or src-1 escalated): from the sibling wave's row
       // src-1 of the other direction, or through an entry whose own row holds src-1
-      if (!chained && src > s0 && (carry_ok || prev_esc)) {
+      if (!pre && !chained && src > s0 && (carry_ok || prev_esc)) {
         // Mirrored, the other direction is this one: its own row src-1 answers (src in U_{src-1}).
         int st = sym ? self_st : 3;
         const unsigned long long want = ((unsigned long long)(k ^ 1) << 32) | (uint32_t)src;
@@ -648,10 +651,10 @@ __global__ __launch_bounds__(kLightWaves * 64) __attribute__((amdgpu_waves_per_e
       // an entry u < src (ELL slots) whose row escalated already: X_src contains X_u. A flag
       // written by another wave may not be seen yet; that only costs the attempt. Read here,
       // tested after the source row's pass (which hides the load).
-      const bool held_esc_lane = sm >= 0 && sm < src && gload(M.esc[dir] + sm) != 0;
+      const bool held_esc_lane = !pre && sm >= 0 && sm < src && gload(M.esc[dir] + sm) != 0;
       bool esc = false;
-      if (chained && prev_esc) {
-        esc = true;  // X_src contains the escalated X_{src-1}
+      if (pre || (chained && prev_esc)) {
+        esc = true;  // X_src contains the escalated X_{src-1}, or its subtree is over the limit
       } else {
         const bool carry = chained && carry_ok;
         if (!carry) {

@@ -286,6 +286,8 @@ int gsofa_run_noalloc(int32_t n, int64_t nnz, const int32_t* rowptr, const int32
   M.paired = paired ? 1 : 0;
   M.light_block = cfg.light_block;
   M.light_end = n;
+  M.subtree = nullptr;  // no pre-escalation: the subtree sizes need the host's elimination tree
+  M.pre_size = INT32_MAX;
   M.light_limit = n <= kLightSmallRows ? kLightLimitSmall
                   : cfg.light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
   const int64_t bases[8] = {0, nitems, 0, n, 0, nblocks_sup, 0, nblk};

@@ -67,7 +67,7 @@ bool same_options(const Options& a, const Options& b) {
          a.light_wgs_per_cu == b.light_wgs_per_cu && a.qcap == b.qcap && a.chain_block == b.chain_block &&
          a.pair_directions == b.pair_directions && a.bitmap_in_hbm == b.bitmap_in_hbm && a.cu_share == b.cu_share &&
          a.split_blocks == b.split_blocks && a.symmetric == b.symmetric &&
-         a.early_top == b.early_top;
+         a.early_top == b.early_top && a.pre_escalate == b.pre_escalate;
 }
 
 // Host bytes a request keeps alive (its prepared pattern).

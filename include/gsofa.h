@@ -168,15 +168,22 @@ typedef struct gsofa_options {
   int32_t cu_share;         /* 0 (default): persistent grids sized for every CU; k > 0: for k
                                CUs, so that work the caller runs beside it on other CUs (a
                                stream with a CU mask) keeps its share of the device            */
-  int32_t early_top;        /* the rows from the start of a large matrix's top separator (a
+  int32_t early_top;        /* 1: the rows from the start of a large matrix's top separator (a
                                block cut whose subtree holds half the rows; matrices of at least
                                a quarter of the set's rows, over 262,144 rows, bitmaps in LDS)
                                run as a few fixed workgroup-path items, three per direction over
                                consecutive blocks, in a launch of their own on an internal
                                stream, which starts with the wave path and joins the run before
-                               supernode detection. 0 (default): in counting-only runs (structure
-                               0); 1: also with the structure; -1 never. Results do not depend
-                               on it                                                           */
+                               supernode detection (counting only, about 3 % faster at 659,033
+                               rows when it is dispatched before the wave path fills the device,
+                               about 10 % slower when not: nothing orders the two). 0 (default)
+                               or -1: not. Results do not depend on it                         */
+  int32_t pre_escalate;     /* 0 (default): in matrices without paired items, a row whose
+                               elimination subtree of A + A^T holds more vertices than the hash
+                               path's escalation limit goes to the bitmap path without a
+                               hash-path attempt (a structurally symmetric pattern's reached set
+                               holds the whole subtree; other patterns' nearly always do). -1:
+                               every row is attempted first. Results do not depend on it       */
 } gsofa_options;
 
 typedef struct gsofa_result {

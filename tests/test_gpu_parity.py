@@ -650,7 +650,7 @@ def test_early_top_launch_changes_nothing(case):
     p, opt = _early_case(case)
     say(f"[early top] {case} n={p.n}")
     out = []
-    for early in (1, -1):  # forced on with the structure (automatic only when counting)
+    for early in (1, -1):
         with g.Plan(p, structure=1, early_top=early, **opt) as plan:
             for k in range(3 if early == 1 else 1):
                 plan.run()
@@ -666,8 +666,8 @@ def test_early_top_launch_changes_nothing(case):
                 else:
                     assert et["items"] == 0 and et["edges"] == 0, et
     assert all(o == out[-1] for o in out), [o[0] for o in out]
-    # counting only: automatic, and the counts, flags and totals of the runs above
-    with g.Plan(p, structure=0, **opt) as plan:
+    # counting only: the counts, flags and totals of the runs above
+    with g.Plan(p, structure=0, early_top=1, **opt) as plan:
         plan.run()
         r = plan.result()
         assert plan.early_top()["items"] > 0
@@ -691,7 +691,7 @@ def test_early_top_launch_in_a_batch_graph():
             r = plan.result()
             want.append(tuple(getattr(r, k) for k in keys) + (g.fnv1a_array(plan.nzu()),
                                                               g.fnv1a_array(plan.supernode_flags())))
-    b = g.Batch(pats)  # counting only: the early launch is automatic
+    b = g.Batch(pats, early_top=1)
     try:
         for rep in range(3):
             say(f"[early top batch] replay {rep}")
@@ -703,3 +703,35 @@ def test_early_top_launch_in_a_batch_graph():
             g.device_synchronize()
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("case", ["grid2d_nd_256", "grid3d_14_chain3", "ds_11948", "grid2d_nd_192_mirrored",
+                                  "grid3d_24_paired"])
+def test_pre_escalation_changes_nothing(case):
+    """Rows whose elimination subtree exceeds the escalation limit skip the wave path; results (with
+    the structure) are those of runs that attempt every row, and the oracle's where it is cheap."""
+    if case == "grid2d_nd_256":
+        p, opt = g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=256, ny=256, order=g.ORDER_ND,
+                            drop_side=g.DROP_LOWER, drop_ppm=100000, seed=5), {}
+    elif case == "grid3d_14_chain3":
+        p, opt = g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=14, ny=14, nz=14, order=g.ORDER_ND,
+                            seed=8), {"chain_block": 3}
+    elif case == "ds_11948":
+        p, opt = load_ds(), {}
+    elif case == "grid2d_nd_192_mirrored":
+        p, opt = g.generate(kind=g.GEN_STENCIL, dims=2, points=9, nx=192, ny=192, order=g.ORDER_ND, seed=3), {}
+    else:
+        p, opt = g.generate(kind=g.GEN_STENCIL, dims=3, points=7, nx=24, ny=24, nz=24, order=g.ORDER_ND,
+                            drop_side=g.DROP_UPPER, drop_ppm=80000, seed=9), {"pair_directions": 1}
+    say(f"[pre-escalation] {case} n={p.n}")
+    check(p, label=f"{case} pre-escalation", **opt)
+    out = []
+    for pre in (0, -1):
+        with g.Plan(p, structure=1, pre_escalate=pre, **opt) as plan:
+            plan.run()
+            r = plan.result()
+            up, ui, lp, li = plan.structure()
+            out.append(((r.fill_count, r.nnz_F, r.nnz_L, r.nnz_U, r.edge_checks, r.reach, r.supernodes_counted),
+                        oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
+                        g.fnv1a_array(plan.supernode_flags())))
+    assert out[0] == out[1]
