@@ -344,7 +344,10 @@ int gsofa_server_submit(gsofa_server* srv, const gsofa_csr* a, const gsofa_optio
   r->opt.chain_block = o->chain_block > 0 ? o->chain_block : 0;
   r->opt.pair_directions = o->pair_directions < 0 ? -1 : (o->pair_directions ? 1 : 0);
   r->opt.bitmap_in_hbm = o->bitmap_in_hbm ? 1 : 0;
-  r->opt.split_blocks = -1;  // the cuts need the elimination tree: not on the admission path
+  // The cuts and pre-escalation need the elimination tree (as much host time as the preparation
+  // itself): not on the admission path, whose host work bounds the served rate.
+  r->opt.split_blocks = -1;
+  r->opt.pre_escalate = -1;
   r->opt.structure = 0;
   DeviceInfo info;
   int rc = device_info(srv->device, info);

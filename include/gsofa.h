@@ -183,7 +183,9 @@ typedef struct gsofa_options {
                                path's escalation limit goes to the bitmap path without a
                                hash-path attempt (a structurally symmetric pattern's reached set
                                holds the whole subtree; other patterns' nearly always do). -1:
-                               every row is attempted first. Results do not depend on it       */
+                               every row is attempted first. The server does not use it (the
+                               elimination tree costs as much host time as the preparation).
+                               Results do not depend on it                                     */
 } gsofa_options;
 
 typedef struct gsofa_result {
