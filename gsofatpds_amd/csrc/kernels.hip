@@ -2194,32 +2194,54 @@ __device__ __forceinline__ int32_t nzu_at(const MatDev& M, int32_t r) {
   return M.cnt[0][r] + (M.dup_u ? M.dup_u[r] : 0);
 }
 
-__device__ __forceinline__ bool is_start(const MatDev& M, int32_t r) {
-  const int32_t off = r % M.chunk;
-  if ((off & 127) == 0) return true;
-  return nzu_at(M, r - 1) - nzu_at(M, r) != 1;
-}
-
 __global__ __launch_bounds__(256) void k_supernodes(const MatDev* __restrict__ mats, int nmats,
                                                     const int64_t* __restrict__ blk_base) {
-  // blk_base[b] = first block of matrix b; each block covers 256 rows of one matrix
+  // blk_base[b] = first block of matrix b; each block covers 256 rows r0 .. r0 + 255 of one
+  // matrix. A row's supernode start lies at most 127 rows before it (row r - ((r mod S) & 127)
+  // starts one), so the block reads the counts of the 129 rows before r0 as well, marks the
+  // starts of rows r0 - 128 .. r0 + 255 as LDS bits and finds each row's start with a few word
+  // scans (a walk row by row through global counts chains up to 127 loads).
+  constexpr int kBack = 128;
+  __shared__ int32_t s_nzu[kBack + 1 + 256];      // rows r0 - 129 .. r0 + 255
+  __shared__ uint32_t s_st[(kBack + 256) / 32];  // start bits of rows r0 - 128 .. r0 + 255
   __shared__ uint32_t s_sum;
-  if (threadIdx.x == 0) s_sum = 0;
-  __syncthreads();
+  const int tid = threadIdx.x;
+  if (tid == 0) s_sum = 0;
+  if (tid < (kBack + 256) / 32) s_st[tid] = 0;
   const int mb = find_segment(blk_base, nmats, (int64_t)blockIdx.x);
   const MatDev& M = mats[mb];
-  const int64_t r64 = ((int64_t)blockIdx.x - blk_base[mb]) * 256 + threadIdx.x;
+  const int32_t n = M.n;
+  const int32_t r0 = (int32_t)(((int64_t)blockIdx.x - blk_base[mb]) * 256);
+  for (int i = tid; i < kBack + 1 + 256; i += 256) {
+    const int32_t row = r0 - kBack - 1 + i;
+    s_nzu[i] = (row >= 0 && row < n) ? nzu_at(M, row) : 0;
+  }
+  __syncthreads();
+  for (int j = tid; j < kBack + 256; j += 256) {
+    const int32_t row = r0 - kBack + j;  // its count at s_nzu[j + 1], row - 1's at s_nzu[j]
+    if (row < 0 || row >= n) continue;
+    if (((row % M.chunk) & 127) == 0 || s_nzu[j] - s_nzu[j + 1] != 1) atomicOr(&s_st[j >> 5], 1u << (j & 31));
+  }
+  __syncthreads();
   uint32_t flag = 0;
-  if (r64 < M.n) {
-    const int32_t r = (int32_t)r64;
-    if (is_start(M, r)) {
+  const int32_t r = r0 + tid;
+  if (r < n) {
+    const int j = kBack + tid;
+    if ((s_st[j >> 5] >> (j & 31)) & 1u) {
       flag = 1;
     } else {
-      int32_t s = r - 1;
-      while (!is_start(M, s)) --s;
-      const int32_t t = r - s - 1;
-      const uint32_t word = M.win[(int64_t)s * 4 + (t >> 5)];
-      flag = ((word >> (t & 31)) & 1u) ? 0u : 1u;
+      // the last start before r: the highest set bit below j (at least j - 127 >= 1)
+      int w = j >> 5;
+      uint32_t bits = s_st[w] & ((1u << (j & 31)) - 1u);
+      while (bits == 0 && w > 0) bits = s_st[--w];
+      if (bits == 0) {
+        flag = 1;  // cannot happen (a start lies within 127 rows); keeps every read in bounds
+      } else {
+        const int32_t s = r0 - kBack + w * 32 + 31 - __clz(bits);
+        const int32_t t = r - s - 1;
+        const uint32_t word = M.win[(int64_t)s * 4 + (t >> 5)];
+        flag = ((word >> (t & 31)) & 1u) ? 0u : 1u;
+      }
     }
     M.sup_flag[r] = (uint8_t)flag;
   }
