@@ -676,6 +676,7 @@ int64_t DeviceSet::footprint(const std::vector<const Prepared*>& mats, const Opt
   b += 4 * (heavy_grid + kTopMaxItems) * heavy_region_ints(nmax, paired);  // spill (and the early launch's)
   if (place_bitmaps_in_hbm(nmax, opt, info)) b += 4 * (int64_t)cus * (((int64_t)nmax + 31) / 32);  // one per CU
   if (dup) b += 8 * nrows;
+  if (opt.pre_escalate != -1) b += 4 * nrows;  // subtree sizes (at most one per row)
   if (opt.structure) b += 16 * (nrows + (int64_t)mats.size());  // offsets; pools are sized later
   b += 1 << 20;                       // tables, accumulators, scan scratch
   *out_bytes = b;
