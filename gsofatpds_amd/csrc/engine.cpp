@@ -119,6 +119,19 @@ int64_t block_cuts(const Prepared& P, int32_t chain_block, int32_t divisor, std:
   return count;
 }
 
+// Share of off-diagonal entries (i, j) whose (j, i) is present, over every 8th row (rows are sorted).
+static double symmetric_share(const Prepared& P) {
+  int64_t seen = 0, matched = 0;
+  for (int32_t v = 0; v < P.n; v += 8)
+    for (int32_t k = P.rowptr[v]; k < P.rowptr[v + 1]; ++k) {
+      const int32_t c = P.adj[k];
+      if (c == v) continue;
+      ++seen;
+      matched += std::binary_search(P.adj.begin() + P.rowptr[c], P.adj.begin() + P.rowptr[c + 1], v) ? 1 : 0;
+    }
+  return seen ? (double)matched / (double)seen : 1.0;
+}
+
 int32_t light_limit_for(int32_t n, int light_bits) {
   return n <= kLightSmallRows ? kLightLimitSmall : light_bits == kLightBitsBig ? kLightLimitBig : kLightLimitMax;
 }
@@ -539,7 +552,9 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
   // escalation limit goes to the workgroup path without a wave-path attempt. A mirrored matrix's
   // reached set holds the whole subtree, so the rule is exact there; a directed reach can be
   // smaller, but on the unsymmetric stencils it sent 11 of 98,731 (659,033 rows) and 53 of
-  // 76,125 (110,592 rows) workgroup-path traversals there needlessly. Paired matrices
+  // 76,125 (110,592 rows) workgroup-path traversals there needlessly. Patterns far from
+  // symmetric (under kPreSymmetry of the sampled entries matched: one direction of a band
+  // reaches almost nothing below its rows, whatever the subtree) and paired matrices
   // (unstructured: reaches far smaller than their subtrees) keep every attempt. Only matrices
   // with more rows than the limit carry the subtree sizes.
   std::vector<int32_t> pre_size((size_t)nmats_, INT32_MAX);
@@ -549,6 +564,7 @@ int DeviceSet::create(const std::vector<const Prepared*>& mats, const Options& o
       if (!mirror_[b] && pair_directions(*mats[b], opt_.pair_directions)) continue;
       const int32_t limit = light_limit_for(ns_[b], cfg_.light_bits);
       if (ns_[b] <= limit) continue;
+      if (!mirror_[b] && symmetric_share(*mats[b]) < kPreSymmetry) continue;
       pre_size[b] = limit;
       any_pre = true;
     }

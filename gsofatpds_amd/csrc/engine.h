@@ -44,6 +44,9 @@ constexpr int32_t kCutShare = 4;
 // escalation flags, so with Options::early_top they run in a launch of their own that starts
 // beside the wave path.
 int32_t top_separator_rows(const Prepared& P, const std::vector<int32_t>& cut);
+// Pre-escalation applies to unmirrored matrices whose off-diagonal entries (sampled) have their
+// transpose at least this often (the measured stencils: 85-90 %).
+constexpr double kPreSymmetry = 0.75;
 // The wave path's escalation limit for a matrix of n rows under a hash-set variant.
 int32_t light_limit_for(int32_t n, int light_bits);
 // At most this many items go to the early launch (one workgroup each), and this many per

@@ -178,11 +178,13 @@ typedef struct gsofa_options {
                                rows when it is dispatched before the wave path fills the device,
                                about 10 % slower when not: nothing orders the two). 0 (default)
                                or -1: not. Results do not depend on it                         */
-  int32_t pre_escalate;     /* 0 (default): in matrices without paired items, a row whose
-                               elimination subtree of A + A^T holds more vertices than the hash
-                               path's escalation limit goes to the bitmap path without a
-                               hash-path attempt (a structurally symmetric pattern's reached set
-                               holds the whole subtree; other patterns' nearly always do). -1:
+  int32_t pre_escalate;     /* 0 (default): in matrices without paired items whose pattern is
+                               symmetric or nearly (three quarters of the sampled off-diagonal
+                               entries have their transpose), a row whose elimination subtree of
+                               A + A^T holds more vertices than the hash path's escalation limit
+                               goes to the bitmap path without a hash-path attempt (a symmetric
+                               pattern's reached set holds the whole subtree; a nearly symmetric
+                               one's nearly always does). -1:
                                every row is attempted first. The server does not use it (the
                                elimination tree costs as much host time as the preparation).
                                Results do not depend on it                                     */

@@ -735,3 +735,25 @@ def test_pre_escalation_changes_nothing(case):
                         oracle.structure_digests(up, ui, lp, li), g.fnv1a_array(plan.nzu()),
                         g.fnv1a_array(plan.supernode_flags())))
     assert out[0] == out[1]
+
+
+def test_pre_escalation_leaves_unsymmetric_bands_alone():
+    """A lower band: direction 0 reaches every row below, direction 1 almost nothing, while every
+    row's elimination subtree is huge. Pre-escalation must not send direction 1's rows to the
+    workgroup path (the pattern is far from symmetric); results equal the oracle's."""
+    n = 20000
+    beg, adj = [], []
+    for i in range(n):
+        beg.append(len(adj))
+        adj.extend(j for j in (i - 3, i - 2, i - 1, i) if j >= 0)
+    beg = np.array(beg, np.int32)
+    end = np.append(beg[1:], len(adj)).astype(np.int32)
+    p = g.Pattern(n, beg, end, np.array(adj, np.int32))
+    check(p, label="lower band", structure=False)
+    hs = []
+    for pre in (0, -1):
+        with g.Plan(p, pre_escalate=pre) as plan:
+            plan.run()
+            hs.append(plan.result().heavy_sources)
+    say(f"[band] workgroup-path traversals with / without pre-escalation: {hs}")
+    assert hs[0] <= 1.05 * hs[1] + 64, hs
